@@ -1,0 +1,403 @@
+// C ABI of libgsr (include/gsr.h): context, buffers, the single-view operators
+// that mirror the pipelines of trainer/gpu.js (createRenderer, gpu.js:704-980)
+// and the batched multi-view step.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "gsr_internal.h"
+
+using namespace gsr;
+
+struct gsr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int num_cus = 0;
+  // Buffers reused between calls, grown by 1.25x (as the reference's scratch
+  // pool, gpu.js:764-772).
+  std::map<std::string, std::pair<void*, size_t>> pool;
+  // Single-view work list, rebuilt when the resolution changes.
+  uint32_t work_w = 0, work_h = 0;
+  // Pinned staging for small device->host reads.
+  uint64_t* pinned = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(gsr_ctx* ctx, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  else g_create_error = buf;
+  return code;
+}
+
+#define GSR_HIP(ctx, call)                                                                        \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess) return fail((ctx), GSR_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define GSR_LAUNCHED(ctx)                                                                              \
+  do {                                                                                                 \
+    hipError_t e_ = hipGetLastError();                                                                 \
+    if (e_ != hipSuccess) return fail((ctx), GSR_E_HIP, "kernel launch: %s", hipGetErrorString(e_)); \
+  } while (0)
+
+// A pooled device buffer of at least `bytes` (contents undefined).
+int pooled(gsr_ctx* ctx, const char* name, size_t bytes, void** out) {
+  auto it = ctx->pool.find(name);
+  if (it != ctx->pool.end() && it->second.second >= bytes) {
+    *out = it->second.first;
+    return GSR_OK;
+  }
+  if (it != ctx->pool.end()) {
+    GSR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    GSR_HIP(ctx, hipFree(it->second.first));
+    ctx->pool.erase(it);
+  }
+  size_t sz = std::max<size_t>(256, ((bytes + bytes / 4) + 255) / 256 * 256);
+  void* p = nullptr;
+  if (hipMalloc(&p, sz) != hipSuccess) return fail(ctx, GSR_E_NOMEM, "out of device memory (%zu bytes for %s)", sz, name);
+  ctx->pool[name] = {p, sz};
+  *out = p;
+  return GSR_OK;
+}
+
+template <class T>
+int pooled_t(gsr_ctx* ctx, const char* name, size_t count, T** out) {
+  void* p = nullptr;
+  int rc = pooled(ctx, name, count * sizeof(T), &p);
+  *out = static_cast<T*>(p);
+  return rc;
+}
+
+ViewDesc view_of(const gsr_camera& c, uint32_t tile_base, uint64_t pixel_base, uint64_t splat_base) {
+  ViewDesc v{};
+  v.width = uint32_t(c.width);
+  v.height = uint32_t(c.height);
+  v.tiles_x = (v.width + kTile - 1) / kTile;
+  v.tiles_y = (v.height + kTile - 1) / kTile;
+  v.tile_base = tile_base;
+  v.num_tiles = v.tiles_x * v.tiles_y;
+  v.pixel_base = pixel_base;
+  v.splat_base = splat_base;
+  v.entry_base = 0;
+  return v;
+}
+
+int check_camera(gsr_ctx* ctx, const gsr_camera* cam) {
+  if (!cam) return fail(ctx, GSR_E_INVALID, "camera is NULL");
+  if (!(cam->width >= 1.0f && cam->height >= 1.0f && cam->width <= 65536.0f && cam->height <= 65536.0f) ||
+      cam->width != float(uint32_t(cam->width)) || cam->height != float(uint32_t(cam->height)))
+    return fail(ctx, GSR_E_INVALID, "camera size %gx%g is not a positive integer size", cam->width, cam->height);
+  return GSR_OK;
+}
+
+// Uploads the single-view descriptor and camera; rebuilds the work list when the
+// resolution changed.
+int single_view(gsr_ctx* ctx, const gsr_camera* cam, ViewDesc** dview, gsr_camera** dcam, TileWork** work,
+                ViewDesc* host) {
+  int rc = check_camera(ctx, cam);
+  if (rc) return rc;
+  *host = view_of(*cam, 0, 0, 0);
+  if ((rc = pooled_t(ctx, "view", 1, dview))) return rc;
+  if ((rc = pooled_t(ctx, "cam", 1, dcam))) return rc;
+  GSR_HIP(ctx, hipMemcpyAsync(*dview, host, sizeof(ViewDesc), hipMemcpyHostToDevice, ctx->stream));
+  GSR_HIP(ctx, hipMemcpyAsync(*dcam, cam, sizeof(gsr_camera), hipMemcpyHostToDevice, ctx->stream));
+  if (work) {
+    if ((rc = pooled_t(ctx, "work", host->num_tiles, work))) return rc;
+    if (ctx->work_w != host->width || ctx->work_h != host->height) {
+      launch_fill_work(ctx->stream, *dview, 1, host->num_tiles, *work);
+      GSR_LAUNCHED(ctx);
+      ctx->work_w = host->width;
+      ctx->work_h = host->height;
+    }
+  }
+  return GSR_OK;
+}
+
+uint32_t pick_chunk(uint64_t items) {
+  // Aim for about 1024 binning blocks; chunks between 256 and 4096 Gaussians.
+  uint64_t c = 256;
+  while (c < 4096 && c * 1024 < items) c <<= 1;
+  return uint32_t(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsr_create(int device, gsr_ctx** out) {
+  if (!out) return fail(nullptr, GSR_E_INVALID, "out is NULL");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0)
+    return fail(nullptr, GSR_E_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= count) return fail(nullptr, GSR_E_INVALID, "device %d out of range [0, %d)", device, count);
+  if ((e = hipSetDevice(device)) != hipSuccess) return fail(nullptr, GSR_E_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  gsr_ctx* ctx = new gsr_ctx();
+  ctx->device = device;
+  if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess) {
+    delete ctx;
+    return fail(nullptr, GSR_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  if (ctx->num_cus <= 0) ctx->num_cus = 256;
+  if ((e = hipHostMalloc(reinterpret_cast<void**>(&ctx->pinned), 4096, hipHostMallocDefault)) != hipSuccess) {
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return fail(nullptr, GSR_E_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
+  }
+  *out = ctx;
+  return GSR_OK;
+}
+
+void gsr_destroy(gsr_ctx* ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->pool) hipFree(kv.second.first);
+  if (ctx->pinned) hipHostFree(ctx->pinned);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* gsr_last_error(const gsr_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int gsr_alloc(gsr_ctx* ctx, size_t bytes, void** dptr) {
+  if (!ctx || !dptr) return fail(ctx, GSR_E_INVALID, "NULL argument");
+  const size_t sz = (std::max<size_t>(16, bytes) + 15) / 16 * 16;
+  void* p = nullptr;
+  if (hipMalloc(&p, sz) != hipSuccess) return fail(ctx, GSR_E_NOMEM, "out of device memory (%zu bytes)", sz);
+  launch_zero(ctx->stream, p, sz);
+  GSR_LAUNCHED(ctx);
+  GSR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *dptr = p;
+  return GSR_OK;
+}
+
+int gsr_free(gsr_ctx* ctx, void* dptr) {
+  if (!ctx) return GSR_E_INVALID;
+  if (!dptr) return GSR_OK;
+  GSR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  GSR_HIP(ctx, hipFree(dptr));
+  return GSR_OK;
+}
+
+int gsr_upload(gsr_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(ctx, GSR_E_INVALID, "NULL argument");
+  if (bytes == 0) return GSR_OK;
+  GSR_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  GSR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GSR_OK;
+}
+
+int gsr_download(gsr_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(ctx, GSR_E_INVALID, "NULL argument");
+  if (bytes == 0) return GSR_OK;
+  GSR_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  GSR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GSR_OK;
+}
+
+int gsr_clear(gsr_ctx* ctx, void* dptr, size_t bytes) {
+  if (!ctx || (!dptr && bytes)) return fail(ctx, GSR_E_INVALID, "NULL argument");
+  if (bytes % 4) return fail(ctx, GSR_E_INVALID, "clear size %zu is not a multiple of 4", bytes);
+  launch_zero(ctx->stream, dptr, bytes);
+  GSR_LAUNCHED(ctx);
+  return GSR_OK;
+}
+
+int gsr_synchronize(gsr_ctx* ctx) {
+  if (!ctx) return GSR_E_INVALID;
+  GSR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GSR_OK;
+}
+
+int gsr_project(gsr_ctx* ctx, const gsr_camera* cam, const float* gs, const float* sh, int sh_degree, uint32_t n,
+                float* splats) {
+  if (!ctx) return GSR_E_INVALID;
+  if (n && (!gs || !splats)) return fail(ctx, GSR_E_INVALID, "gs / splats is NULL");
+  if (sh_degree < 0 || sh_degree > 3) return fail(ctx, GSR_E_INVALID, "sh_degree %d not in [0, 3]", sh_degree);
+  if (sh_degree > 0 && !sh) return fail(ctx, GSR_E_INVALID, "sh is NULL with sh_degree %d", sh_degree);
+  ViewDesc* dview;
+  gsr_camera* dcam;
+  ViewDesc host;
+  int rc = single_view(ctx, cam, &dview, &dcam, nullptr, &host);
+  if (rc) return rc;
+  launch_project(ctx->stream, dcam, dview, 1, gs, sh, sh_degree, n, splats, nullptr, nullptr, nullptr);
+  GSR_LAUNCHED(ctx);
+  return GSR_OK;
+}
+
+int gsr_bin_sort(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, uint32_t n, uint32_t* order,
+                 uint64_t order_capacity, uint32_t* ranges, uint64_t* num_entries) {
+  if (!ctx) return GSR_E_INVALID;
+  if (!ranges || !num_entries || (n && !splats)) return fail(ctx, GSR_E_INVALID, "NULL argument");
+  if (order_capacity && !order) return fail(ctx, GSR_E_INVALID, "order is NULL with capacity %llu",
+                                            (unsigned long long)order_capacity);
+  ViewDesc* dview;
+  gsr_camera* dcam;
+  ViewDesc host;
+  int rc = single_view(ctx, cam, &dview, &dcam, nullptr, &host);
+  if (rc) return rc;
+  const size_t nn = std::max<uint32_t>(n, 1);
+  uint32_t *keys, *keys_alt, *vals, *vals_alt, *hist, *piece, *matrix, *tile_tot, *overflow;
+  uint2* rects;
+  uint64_t *matrix_off, *entries;
+  const uint32_t chunk = pick_chunk(n);
+  const size_t hist_n = std::max<size_t>(1, radix_hist_elems(n, 1));
+  const size_t piece_n = std::max<size_t>(1, scan_piece_elems(hist_n, 1));
+  const size_t mat_n = std::max<size_t>(1, bin_matrix_elems(&host, 1, n, chunk));
+  if ((rc = pooled_t(ctx, "keys", nn, &keys)) || (rc = pooled_t(ctx, "keys_alt", nn, &keys_alt)) ||
+      (rc = pooled_t(ctx, "vals", nn, &vals)) || (rc = pooled_t(ctx, "vals_alt", nn, &vals_alt)) ||
+      (rc = pooled_t(ctx, "rects", nn, &rects)) || (rc = pooled_t(ctx, "hist", hist_n, &hist)) ||
+      (rc = pooled_t(ctx, "piece", piece_n, &piece)) || (rc = pooled_t(ctx, "matrix", mat_n, &matrix)) ||
+      (rc = pooled_t(ctx, "matrix_off", 1, &matrix_off)) ||
+      (rc = pooled_t(ctx, "tile_tot", host.num_tiles, &tile_tot)) || (rc = pooled_t(ctx, "entries", 1, &entries)) ||
+      (rc = pooled_t(ctx, "overflow", 1, &overflow)))
+    return rc;
+  launch_zero(ctx->stream, matrix_off, sizeof(uint64_t));
+  launch_zero(ctx->stream, overflow, sizeof(uint32_t));
+  launch_prep(ctx->stream, dview, 1, splats, n, keys, rects);
+  SortScratch sc{keys_alt, vals, vals_alt, hist, piece};
+  launch_depth_sort(ctx->stream, keys, n, 1, sc);
+  BinScratch bs{matrix, matrix_off, tile_tot, entries};
+  launch_bin_count(ctx->stream, dview, 1, n, chunk, keys, vals, rects, bs, ranges, host.num_tiles);
+  GSR_LAUNCHED(ctx);
+  GSR_HIP(ctx, hipMemcpyAsync(ctx->pinned, entries, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  GSR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const uint64_t E = ctx->pinned[0];
+  *num_entries = E;
+  if (E > order_capacity)
+    return fail(ctx, GSR_E_CAPACITY, "order holds %llu entries, %llu needed", (unsigned long long)order_capacity,
+                (unsigned long long)E);
+  launch_bin_emit(ctx->stream, dview, 1, n, chunk, keys, vals, rects, bs, ranges, order, order_capacity, overflow,
+                  host.num_tiles);
+  GSR_LAUNCHED(ctx);
+  return GSR_OK;
+}
+
+static int raster_common(gsr_ctx* ctx, const gsr_camera* cam, RasterArgs& a, ViewDesc& host) {
+  ViewDesc* dview;
+  gsr_camera* dcam;
+  TileWork* work;
+  int rc = single_view(ctx, cam, &dview, &dcam, &work, &host);
+  if (rc) return rc;
+  a.views = dview;
+  a.work = work;
+  a.num_work = host.num_tiles;
+  a.grads_view_stride = 0;
+  a.queue = nullptr;
+  a.tile_batch = 1;
+  return GSR_OK;
+}
+
+int gsr_forward(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, const uint32_t* order,
+                const uint32_t* ranges, float* image, float* state) {
+  if (!ctx) return GSR_E_INVALID;
+  if (!splats || !order || !ranges || !image) return fail(ctx, GSR_E_INVALID, "NULL argument");
+  RasterArgs a{};
+  ViewDesc host;
+  int rc = raster_common(ctx, cam, a, host);
+  if (rc) return rc;
+  a.splats = splats;
+  a.order = order;
+  a.ranges = ranges;
+  a.image = image;
+  a.state = state;
+  launch_forward(ctx->stream, a, 0, 0);
+  GSR_LAUNCHED(ctx);
+  return GSR_OK;
+}
+
+int gsr_loss_grad(gsr_ctx* ctx, const gsr_camera* cam, const float* image, const float* target, float lambda,
+                  float* loss_grad) {
+  if (!ctx) return GSR_E_INVALID;
+  if (!image || !target || !loss_grad) return fail(ctx, GSR_E_INVALID, "NULL argument");
+  if (!(lambda >= 0.0f && lambda <= 1.0f)) return fail(ctx, GSR_E_INVALID, "lambda %g not in [0, 1]", lambda);
+  ViewDesc* dview;
+  gsr_camera* dcam;
+  ViewDesc host;
+  int rc = single_view(ctx, cam, &dview, &dcam, nullptr, &host);
+  if (rc) return rc;
+  float* coef = nullptr;
+  if (lambda > 0.0f && (rc = pooled_t(ctx, "ssim_coef", size_t(host.width) * host.height * 9, &coef))) return rc;
+  launch_loss_grad(ctx->stream, dview, &host, 1, image, target, lambda, coef, loss_grad);
+  GSR_LAUNCHED(ctx);
+  return GSR_OK;
+}
+
+int gsr_backward(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, const uint32_t* order,
+                 const uint32_t* ranges, const float* image, const float* loss_grad, float* grads) {
+  if (!ctx) return GSR_E_INVALID;
+  if (!splats || !order || !ranges || !image || !loss_grad || !grads)
+    return fail(ctx, GSR_E_INVALID, "NULL argument");
+  RasterArgs a{};
+  ViewDesc host;
+  int rc = raster_common(ctx, cam, a, host);
+  if (rc) return rc;
+  a.splats = splats;
+  a.order = order;
+  a.ranges = ranges;
+  a.image = const_cast<float*>(image);
+  a.dl = loss_grad;
+  a.grads = grads;
+  launch_backward(ctx->stream, a, 0, 0);
+  GSR_LAUNCHED(ctx);
+  return GSR_OK;
+}
+
+int gsr_shape(gsr_ctx* ctx, const gsr_camera* cam, const float* gs, const float* sh, int sh_degree, uint32_t n,
+              const float* grads, float* shape, float* sh_grad) {
+  if (!ctx) return GSR_E_INVALID;
+  if (n && (!gs || !grads || !shape)) return fail(ctx, GSR_E_INVALID, "NULL argument");
+  if (sh_degree < 0 || sh_degree > 3) return fail(ctx, GSR_E_INVALID, "sh_degree %d not in [0, 3]", sh_degree);
+  if (sh_degree > 0 && (!sh || !sh_grad)) return fail(ctx, GSR_E_INVALID, "sh / sh_grad is NULL");
+  ViewDesc* dview;
+  gsr_camera* dcam;
+  ViewDesc host;
+  int rc = single_view(ctx, cam, &dview, &dcam, nullptr, &host);
+  if (rc) return rc;
+  launch_shape(ctx->stream, dcam, 1, gs, sh, sh_degree, n, const_cast<float*>(grads), 0, shape, sh_grad, nullptr, 0);
+  GSR_LAUNCHED(ctx);
+  return GSR_OK;
+}
+
+int gsr_adam(gsr_ctx* ctx, const gsr_step* step, float* gs, const float* grads, const float* shape,
+             float* moments) {
+  if (!ctx) return GSR_E_INVALID;
+  if (!step || (step->count && (!gs || !grads || !shape || !moments))) return fail(ctx, GSR_E_INVALID, "NULL argument");
+  launch_adam(ctx->stream, *step, gs, grads, shape, moments);
+  GSR_LAUNCHED(ctx);
+  return GSR_OK;
+}
+
+int gsr_stats(gsr_ctx* ctx, const gsr_step* step, const float* grads, const float* splats, float* stats) {
+  if (!ctx) return GSR_E_INVALID;
+  if (!step || (step->count && (!grads || !splats || !stats))) return fail(ctx, GSR_E_INVALID, "NULL argument");
+  launch_stats(ctx->stream, *step, grads, splats, stats);
+  GSR_LAUNCHED(ctx);
+  return GSR_OK;
+}
+
+}  // extern "C"
+
+namespace gsr {
+hipStream_t ctx_stream(gsr_ctx* ctx) { return ctx->stream; }
+int ctx_num_cus(gsr_ctx* ctx) { return ctx->num_cus; }
+int ctx_fail(gsr_ctx* ctx, int code, const char* msg) { return fail(ctx, code, "%s", msg); }
+}  // namespace gsr

@@ -1,0 +1,390 @@
+// Batched views: one shared Gaussian set rendered and differentiated from many
+// cameras of any resolution per step (the reference renders one view per step,
+// trainer/gpu.js:915-933).
+//
+// Step = project (all views, one launch) -> per-view depth radix sort (one
+// segmented launch per pass) -> tile binning (count, scan, emit) -> persistent
+// forward over a work queue of every view's tiles -> loss gradient -> persistent
+// backward -> chain to the 3D parameters summed over views.  The sequence is
+// captured once into a HIP graph and replayed; every size it launches with is
+// fixed by the cameras except the entry count, which is bounded by the order
+// capacity: an overflowing step is detected, storage grows and the step reruns.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "gsr_internal.h"
+
+using namespace gsr;
+
+struct gsr_ctx;  // defined in gsr_api.hip
+extern "C" const char* gsr_last_error(const gsr_ctx* ctx);
+
+namespace gsr {
+// Accessors into the context (gsr_api.hip).
+hipStream_t ctx_stream(gsr_ctx* ctx);
+int ctx_num_cus(gsr_ctx* ctx);
+int ctx_fail(gsr_ctx* ctx, int code, const char* msg);
+}  // namespace gsr
+
+struct gsr_batch {
+  gsr_ctx* ctx = nullptr;
+  uint32_t V = 0, n = 0;
+  int deg = 0;
+  std::vector<gsr_camera> cams;
+  std::vector<ViewDesc> views;
+  uint32_t total_tiles = 0, max_tiles = 0;
+  uint64_t total_pixels = 0;
+  uint32_t chunk = 0, tile_batch = 1;
+  uint32_t grid_fwd = 0, grid_bwd = 0;
+  // device
+  gsr_camera* d_cams = nullptr;
+  ViewDesc* d_views = nullptr;
+  TileWork* d_work = nullptr;
+  float* splats = nullptr;
+  uint32_t *keys = nullptr, *keys_alt = nullptr, *vals = nullptr, *vals_alt = nullptr;
+  uint2* rects = nullptr;
+  uint32_t *hist = nullptr, *piece = nullptr;
+  uint32_t* matrix = nullptr;
+  uint64_t* matrix_off = nullptr;
+  uint32_t* tile_tot = nullptr;
+  uint64_t* entries = nullptr;
+  uint32_t* ranges = nullptr;
+  uint32_t* order = nullptr;
+  uint64_t order_cap = 0;
+  uint32_t* flags = nullptr;  // [0] overflow, [1] forward queue, [2] backward queue
+  float* image = nullptr;
+  float* dl = nullptr;
+  float* coef = nullptr;
+  float* vgrads = nullptr;
+  uint64_t* pinned = nullptr;  // host: entries[V], overflow
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<const void*> graph_key;
+  float graph_lambda = -1.0f;
+};
+
+namespace {
+
+void dfree(void* p) {
+  if (p) hipFree(p);
+}
+
+template <class T>
+int dalloc(gsr_batch* b, T** p, size_t count) {
+  dfree(*p);
+  *p = nullptr;
+  size_t bytes = std::max<size_t>(256, count * sizeof(T));
+  if (hipMalloc(reinterpret_cast<void**>(p), bytes) != hipSuccess) {
+    *p = nullptr;
+    return ctx_fail(b->ctx, GSR_E_NOMEM, "out of device memory for a batch buffer");
+  }
+  launch_zero(ctx_stream(b->ctx), *p, bytes / 4 * 4);
+  return GSR_OK;
+}
+
+void drop_graph(gsr_batch* b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_key.clear();
+}
+
+int layout(gsr_batch* b) {
+  b->views.resize(b->V);
+  uint32_t tb = 0;
+  uint64_t pb = 0;
+  b->max_tiles = 0;
+  for (uint32_t v = 0; v < b->V; ++v) {
+    const gsr_camera& c = b->cams[v];
+    ViewDesc& d = b->views[v];
+    d = ViewDesc{};
+    d.width = uint32_t(c.width);
+    d.height = uint32_t(c.height);
+    d.tiles_x = (d.width + kTile - 1) / kTile;
+    d.tiles_y = (d.height + kTile - 1) / kTile;
+    d.num_tiles = d.tiles_x * d.tiles_y;
+    d.tile_base = tb;
+    d.pixel_base = pb;
+    d.splat_base = uint64_t(v) * b->n;
+    tb += d.num_tiles;
+    pb += uint64_t(d.width) * d.height;
+    b->max_tiles = std::max(b->max_tiles, d.num_tiles);
+  }
+  b->total_tiles = tb;
+  b->total_pixels = pb;
+  return GSR_OK;
+}
+
+int alloc_all(gsr_batch* b) {
+  const size_t vn = size_t(b->V) * std::max<uint32_t>(b->n, 1);
+  int rc;
+  hipStream_t s = ctx_stream(b->ctx);
+  if ((rc = dalloc(b, &b->d_cams, b->V)) || (rc = dalloc(b, &b->d_views, b->V)) ||
+      (rc = dalloc(b, &b->d_work, b->total_tiles)) || (rc = dalloc(b, &b->splats, vn * 12)) ||
+      (rc = dalloc(b, &b->keys, vn)) || (rc = dalloc(b, &b->keys_alt, vn)) || (rc = dalloc(b, &b->vals, vn)) ||
+      (rc = dalloc(b, &b->vals_alt, vn)) || (rc = dalloc(b, &b->rects, vn)) ||
+      (rc = dalloc(b, &b->hist, std::max<size_t>(1, radix_hist_elems(b->n, b->V)))) ||
+      (rc = dalloc(b, &b->piece,
+                   std::max<size_t>(1, scan_piece_elems(radix_hist_elems(b->n, 1), b->V)))) ||
+      (rc = dalloc(b, &b->matrix_off, b->V)) || (rc = dalloc(b, &b->tile_tot, b->total_tiles)) ||
+      (rc = dalloc(b, &b->entries, b->V)) || (rc = dalloc(b, &b->ranges, size_t(b->total_tiles) * 2)) ||
+      (rc = dalloc(b, &b->flags, 64)) || (rc = dalloc(b, &b->image, b->total_pixels * 3)) ||
+      (rc = dalloc(b, &b->dl, b->total_pixels * 3)) ||
+      (rc = dalloc(b, &b->vgrads, size_t(b->V) * std::max<uint32_t>(b->n, 1) * 16)))
+    return rc;
+  (void)s;
+  return GSR_OK;
+}
+
+int set_chunk(gsr_batch* b, uint32_t chunk) {
+  b->chunk = chunk;
+  const size_t m = bin_matrix_elems(b->views.data(), b->V, b->n, chunk);
+  int rc = dalloc(b, &b->matrix, std::max<size_t>(1, m));
+  if (rc) return rc;
+  std::vector<uint64_t> off(b->V);
+  const uint64_t chunks = (uint64_t(b->n) + chunk - 1) / chunk;
+  uint64_t run = 0;
+  for (uint32_t v = 0; v < b->V; ++v) {
+    off[v] = run;
+    run += chunks * b->views[v].num_tiles;
+  }
+  if (hipMemcpy(b->matrix_off, off.data(), off.size() * sizeof(uint64_t), hipMemcpyHostToDevice) != hipSuccess)
+    return ctx_fail(b->ctx, GSR_E_HIP, "upload of matrix offsets failed");
+  drop_graph(b);
+  return GSR_OK;
+}
+
+int upload_cams(gsr_batch* b) {
+  hipStream_t s = ctx_stream(b->ctx);
+  if (hipMemcpyAsync(b->d_cams, b->cams.data(), b->V * sizeof(gsr_camera), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(b->d_views, b->views.data(), b->V * sizeof(ViewDesc), hipMemcpyHostToDevice, s) != hipSuccess)
+    return ctx_fail(b->ctx, GSR_E_HIP, "camera upload failed");
+  launch_fill_work(s, b->d_views, b->V, b->total_tiles, b->d_work);
+  if (hipStreamSynchronize(s) != hipSuccess) return ctx_fail(b->ctx, GSR_E_HIP, "camera upload failed");
+  return GSR_OK;
+}
+
+// Issues one full step on the stream (capturable: kernels only).
+void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* targets, float lambda, float* grads,
+                float* shape, float* sh_grad, float* images) {
+  hipStream_t s = ctx_stream(b->ctx);
+  float* image = images ? images : b->image;
+  launch_zero(s, b->flags, 16);
+  launch_project(s, b->d_cams, b->d_views, b->V, gs, sh, b->deg, b->n, b->splats, b->keys, b->rects, nullptr);
+  SortScratch sc{b->keys_alt, b->vals, b->vals_alt, b->hist, b->piece};
+  launch_depth_sort(s, b->keys, b->n, b->V, sc);
+  BinScratch bs{b->matrix, b->matrix_off, b->tile_tot, b->entries};
+  launch_bin_count(s, b->d_views, b->V, b->n, b->chunk, b->keys, b->vals, b->rects, bs, b->ranges, b->max_tiles);
+  launch_bin_emit(s, b->d_views, b->V, b->n, b->chunk, b->keys, b->vals, b->rects, bs, b->ranges, b->order,
+                  b->order_cap, b->flags, b->max_tiles);
+  RasterArgs a{};
+  a.views = b->d_views;
+  a.work = b->d_work;
+  a.num_work = b->total_tiles;
+  a.splats = b->splats;
+  a.order = b->order;
+  a.ranges = b->ranges;
+  a.image = image;
+  a.state = nullptr;
+  a.dl = b->dl;
+  a.grads = b->vgrads;
+  a.grads_view_stride = uint64_t(b->n) * 16;
+  a.tile_batch = b->tile_batch;
+  a.queue = b->flags + 1;
+  launch_forward(s, a, 1, b->grid_fwd);
+  launch_loss_grad(s, b->d_views, b->views.data(), b->V, image, targets, lambda, b->coef, b->dl);
+  a.queue = b->flags + 2;
+  launch_backward(s, a, 1, b->grid_bwd);
+  launch_shape(s, b->d_cams, b->V, gs, sh, b->deg, b->n, b->vgrads, uint64_t(b->n) * 16, shape, sh_grad, grads, 1);
+}
+
+int run_step(gsr_batch* b, const float* gs, const float* sh, const float* targets, float lambda, float* grads,
+             float* shape, float* sh_grad, float* images, int use_graph) {
+  hipStream_t s = ctx_stream(b->ctx);
+  if (lambda > 0.0f && !b->coef) {
+    int rc = dalloc(b, &b->coef, b->total_pixels * 9);
+    if (rc) return rc;
+    drop_graph(b);
+  }
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    if (use_graph) {
+      std::vector<const void*> key = {gs, sh, targets, grads, shape, sh_grad, images};
+      if (!b->exec || key != b->graph_key || lambda != b->graph_lambda) {
+        drop_graph(b);
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+          return ctx_fail(b->ctx, GSR_E_HIP, "hipStreamBeginCapture failed");
+        issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, images);
+        hipError_t e = hipStreamEndCapture(s, &b->graph);
+        if (e != hipSuccess) return ctx_fail(b->ctx, GSR_E_HIP, "hipStreamEndCapture failed");
+        if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+          return ctx_fail(b->ctx, GSR_E_HIP, "hipGraphInstantiate failed");
+        b->graph_key = key;
+        b->graph_lambda = lambda;
+      }
+      if (hipGraphLaunch(b->exec, s) != hipSuccess) return ctx_fail(b->ctx, GSR_E_HIP, "hipGraphLaunch failed");
+    } else {
+      issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, images);
+      if (hipGetLastError() != hipSuccess) return ctx_fail(b->ctx, GSR_E_HIP, "kernel launch failed");
+    }
+    if (hipMemcpyAsync(b->pinned, b->entries, b->V * sizeof(uint64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(b->pinned + b->V, b->flags, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess)
+      return ctx_fail(b->ctx, GSR_E_HIP, "readback failed");
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      std::string m = std::string("batch step failed: ") + hipGetErrorString(e);
+      return ctx_fail(b->ctx, GSR_E_HIP, m.c_str());
+    }
+    uint64_t total = 0;
+    for (uint32_t v = 0; v < b->V; ++v) total += b->pinned[v];
+    const bool overflow = (uint32_t(b->pinned[b->V]) & 0xFFFFFFFFu) != 0 || total > b->order_cap;
+    if (!overflow) return GSR_OK;
+    // Grow the entry storage and run the step again.
+    uint64_t cap = total + total / 4 + 1024;
+    int rc = dalloc(b, &b->order, cap);
+    if (rc) return rc;
+    b->order_cap = cap;
+    drop_graph(b);
+  }
+  return ctx_fail(b->ctx, GSR_E_CAPACITY, "entry storage kept overflowing");
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsr_batch_create(gsr_ctx* ctx, uint32_t num_views, const gsr_camera* cams, uint32_t n, int sh_degree,
+                     gsr_batch** out) {
+  if (!ctx) return GSR_E_INVALID;
+  if (!out || !cams || num_views == 0) return ctx_fail(ctx, GSR_E_INVALID, "no cameras");
+  if (sh_degree < 0 || sh_degree > 3) return ctx_fail(ctx, GSR_E_INVALID, "sh_degree not in [0, 3]");
+  for (uint32_t v = 0; v < num_views; ++v) {
+    const gsr_camera& c = cams[v];
+    if (!(c.width >= 1.0f && c.height >= 1.0f && c.width <= 65536.0f && c.height <= 65536.0f) ||
+        c.width != float(uint32_t(c.width)) || c.height != float(uint32_t(c.height)))
+      return ctx_fail(ctx, GSR_E_INVALID, "camera size is not a positive integer size");
+  }
+  gsr_batch* b = new gsr_batch();
+  b->ctx = ctx;
+  b->V = num_views;
+  b->n = n;
+  b->deg = sh_degree;
+  b->cams.assign(cams, cams + num_views);
+  int rc = layout(b);
+  if (!rc) rc = alloc_all(b);
+  uint64_t items = uint64_t(n) * num_views;
+  uint32_t chunk = 256;
+  while (chunk < 4096 && uint64_t(chunk) * 2048 < items) chunk <<= 1;
+  if (!rc) rc = set_chunk(b, chunk);
+  if (!rc && hipHostMalloc(reinterpret_cast<void**>(&b->pinned), (num_views + 8) * sizeof(uint64_t),
+                           hipHostMallocDefault) != hipSuccess)
+    rc = ctx_fail(ctx, GSR_E_HIP, "hipHostMalloc failed");
+  if (!rc) rc = upload_cams(b);
+  if (!rc) {
+    // Persistent grids: every CU filled to the kernels' occupancy.
+    int per_cu_f = 0, per_cu_b = 0;
+    raster_occupancy(&per_cu_f, &per_cu_b);
+    b->grid_fwd = uint32_t(std::max(1, per_cu_f) * ctx_num_cus(ctx));
+    b->grid_bwd = uint32_t(std::max(1, per_cu_b) * ctx_num_cus(ctx));
+    // A first guess of the entry storage: 16 entries per Gaussian and view.
+    uint64_t cap = std::max<uint64_t>(1024, items * 16);
+    rc = dalloc(b, &b->order, cap);
+    b->order_cap = cap;
+  }
+  if (rc) {
+    gsr_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return GSR_OK;
+}
+
+void gsr_batch_destroy(gsr_batch* b) {
+  if (!b) return;
+  hipStreamSynchronize(ctx_stream(b->ctx));
+  drop_graph(b);
+  void* ptrs[] = {b->d_cams, b->d_views, b->d_work, b->splats, b->keys, b->keys_alt, b->vals, b->vals_alt,
+                  b->rects, b->hist, b->piece, b->matrix, b->matrix_off, b->tile_tot, b->entries, b->ranges,
+                  b->order, b->flags, b->image, b->dl, b->coef, b->vgrads};
+  for (void* p : ptrs) dfree(p);
+  if (b->pinned) hipHostFree(b->pinned);
+  delete b;
+}
+
+int gsr_batch_set_cameras(gsr_batch* b, const gsr_camera* cams) {
+  if (!b || !cams) return GSR_E_INVALID;
+  for (uint32_t v = 0; v < b->V; ++v)
+    if (cams[v].width != b->cams[v].width || cams[v].height != b->cams[v].height)
+      return ctx_fail(b->ctx, GSR_E_INVALID, "camera resolution changed");
+  b->cams.assign(cams, cams + b->V);
+  return upload_cams(b);
+}
+
+int gsr_batch_step(gsr_batch* b, const float* gs, const float* sh, const float* targets, float lambda, float* grads,
+                   float* shape, float* sh_grad, float* images, int use_graph) {
+  if (!b) return GSR_E_INVALID;
+  if ((b->n && (!gs || !grads || !shape)) || !targets) return ctx_fail(b->ctx, GSR_E_INVALID, "NULL argument");
+  if (b->deg > 0 && (!sh || !sh_grad)) return ctx_fail(b->ctx, GSR_E_INVALID, "sh / sh_grad is NULL");
+  if (!(lambda >= 0.0f && lambda <= 1.0f)) return ctx_fail(b->ctx, GSR_E_INVALID, "lambda not in [0, 1]");
+  return run_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, images, use_graph);
+}
+
+int gsr_batch_entries(gsr_batch* b, uint64_t* entries) {
+  if (!b || !entries) return GSR_E_INVALID;
+  for (uint32_t v = 0; v < b->V; ++v) entries[v] = b->pinned[v];
+  return GSR_OK;
+}
+
+int gsr_batch_configure(gsr_batch* b, uint32_t tile_batch, uint32_t chunk) {
+  if (!b) return GSR_E_INVALID;
+  if (tile_batch) {
+    if (tile_batch > 256) return ctx_fail(b->ctx, GSR_E_INVALID, "tile_batch above 256");
+    if (tile_batch != b->tile_batch) drop_graph(b);
+    b->tile_batch = tile_batch;
+  }
+  if (chunk && chunk != b->chunk) {
+    if (chunk < 64 || chunk > 65536) return ctx_fail(b->ctx, GSR_E_INVALID, "chunk not in [64, 65536]");
+    return set_chunk(b, chunk);
+  }
+  return GSR_OK;
+}
+
+int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const float* targets, float lambda,
+                       float* grads, float* shape, float* sh_grad, uint32_t* tile_batch, uint32_t* chunk) {
+  if (!b) return GSR_E_INVALID;
+  const uint32_t tbs[] = {1, 2, 4, 8};
+  const uint32_t chunks[] = {1024, 2048, 4096};
+  double best = 1e30;
+  uint32_t best_tb = b->tile_batch, best_ch = b->chunk;
+  for (uint32_t ch : chunks) {
+    for (uint32_t tb : tbs) {
+      int rc = gsr_batch_configure(b, tb, ch);
+      if (rc) return rc;
+      // Warm (captures the graph), then time two replays.
+      rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
+      if (rc) return rc;
+      auto t0 = std::chrono::steady_clock::now();
+      for (int k = 0; k < 2; ++k) {
+        rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
+        if (rc) return rc;
+      }
+      double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (dt < best) {
+        best = dt;
+        best_tb = tb;
+        best_ch = ch;
+      }
+    }
+  }
+  int rc = gsr_batch_configure(b, best_tb, best_ch);
+  if (rc) return rc;
+  if (tile_batch) *tile_batch = best_tb;
+  if (chunk) *chunk = best_ch;
+  return GSR_OK;
+}
+
+}  // extern "C"

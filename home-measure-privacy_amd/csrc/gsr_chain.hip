@@ -1,0 +1,273 @@
+// From screen-space gradients to the 3D parameters: the analytic derivative of
+// projectShape (trainer/gpu.js:30-60), replacing the central differences of
+// SHAPE (gpu.js:297-328), plus the SH colour chain (extension); ADAM
+// (gpu.js:411-443) and STATS (gpu.js:483-497).
+//
+// One thread per Gaussian loops over the views of a launch.  The covariance
+// gradient is linear in the per-view terms, so the views' world-space covariance
+// gradients are summed first and pushed through M = R(q) diag(exp(s)) once.
+#include "gsr_internal.h"
+#include "gsr_math.h"
+
+namespace gsr {
+
+namespace {
+
+__global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict__ cams, uint32_t num_views,
+                                                    const float* __restrict__ gs, const float* __restrict__ sh,
+                                                    int deg, uint32_t n, float* __restrict__ grads,
+                                                    uint64_t stride, float* __restrict__ shape,
+                                                    float* __restrict__ sh_grad, float* __restrict__ grads_out,
+                                                    int consume) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4* g4 = reinterpret_cast<const float4*>(gs) + size_t(i) * 4;
+  const float4 pa = g4[0], pb = g4[1], pc = g4[2];
+  const float mean[3] = {pa.x, pa.y, pa.z};
+  const float q[10] = {pa.x, pa.y, pa.z, pb.x, pb.y, pb.z, pc.x, pc.y, pc.z, pc.w};
+  float S[3][3];
+  world_cov(q, S);
+  float shc[48];
+  float gsh[48];
+  if (deg > 0) {
+    const float4* s4 = reinterpret_cast<const float4*>(sh) + size_t(i) * 12;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      float4 v = s4[k];
+      shc[4 * k] = v.x; shc[4 * k + 1] = v.y; shc[4 * k + 2] = v.z; shc[4 * k + 3] = v.w;
+    }
+#pragma unroll
+    for (int k = 0; k < 48; ++k) gsh[k] = 0.0f;
+  }
+  float Gw[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+  float gmean[3] = {0.f, 0.f, 0.f};
+  float gcol[3] = {0.f, 0.f, 0.f};
+  float gop = 0.0f;
+  for (uint32_t v = 0; v < num_views; ++v) {
+    float4* r = reinterpret_cast<float4*>(grads + size_t(v) * stride + size_t(i) * 16);
+    const float4 ga4 = r[0], gb4 = r[1], gc4 = r[2];
+    const float gr[9] = {ga4.x, ga4.y, ga4.z, ga4.w, gb4.x, gb4.y, gb4.z, gb4.w, gc4.x};
+    gcol[0] += gr[0]; gcol[1] += gr[1]; gcol[2] += gr[2];
+    gop += gr[3];
+    bool screen = gr[4] != 0.0f || gr[5] != 0.0f || gr[6] != 0.0f || gr[7] != 0.0f || gr[8] != 0.0f;
+    bool colour = deg > 0 && (gr[0] != 0.0f || gr[1] != 0.0f || gr[2] != 0.0f);
+    if (consume && (screen || gr[0] != 0.0f || gr[1] != 0.0f || gr[2] != 0.0f || gr[3] != 0.0f)) {
+      const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+      r[0] = z; r[1] = z; r[2] = z;
+    }
+    if (!screen && !colour) continue;
+    const CamF cam = load_cam(cams[v]);
+    float p[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      p[k] = cam.V[k][0] * mean[0] + cam.V[k][1] * mean[1] + cam.V[k][2] * mean[2] + cam.t[k];
+    if (p[2] < 0.2f) continue;
+    // Camera-frame covariance, as project_cov.
+    float T[3][3], C[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) T[a][b] = cam.V[a][0] * S[0][b] + cam.V[a][1] * S[1][b] + cam.V[a][2] * S[2][b];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) C[a][b] = T[a][0] * cam.V[b][0] + T[a][1] * cam.V[b][1] + T[a][2] * cam.V[b][2];
+    const float x = p[0], y = p[1], z = p[2];
+    const float iz = 1.0f / z, iz2 = iz * iz;
+    const float j0[3] = {cam.fx * iz, 0.0f, -cam.fx * x * iz2};
+    const float j1[3] = {0.0f, cam.fy * iz, -cam.fy * y * iz2};
+    float Cj0[3], Cj1[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      Cj0[a] = C[a][0] * j0[0] + C[a][1] * j0[1] + C[a][2] * j0[2];
+      Cj1[a] = C[a][0] * j1[0] + C[a][1] * j1[1] + C[a][2] * j1[2];
+    }
+    const float ca = j0[0] * Cj0[0] + j0[1] * Cj0[1] + j0[2] * Cj0[2] + 0.3f;
+    const float cb = j0[0] * Cj1[0] + j0[1] * Cj1[1] + j0[2] * Cj1[2];
+    const float cc = j1[0] * Cj1[0] + j1[1] * Cj1[1] + j1[2] * Cj1[2] + 0.3f;
+    const float D = ca * cc - cb * cb;
+    if (!(D > 0.0f)) continue;
+    if (screen) {
+      const float gu = gr[4], gv = gr[5], gA = gr[6], gB = gr[7], gC = gr[8];
+      // Conic (A, B, C) = (c, -b, a) / D.
+      const float iD = 1.0f / D, iD2 = iD * iD;
+      const float g_a = (-gA * cc * cc + gB * cb * cc - gC * cb * cb) * iD2;
+      const float g_b = (2.0f * gA * cb * cc + 2.0f * gC * ca * cb - 2.0f * gB * cb * cb) * iD2 - gB * iD;
+      const float g_c = (-gA * cb * cb + gB * ca * cb - gC * ca * ca) * iD2;
+      // dL/dC (camera frame, symmetric) and dL/dj0, dL/dj1.
+      float G[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+          G[a][b] = g_a * j0[a] * j0[b] + 0.5f * g_b * (j0[a] * j1[b] + j1[a] * j0[b]) + g_c * j1[a] * j1[b];
+      float gj0[3], gj1[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        gj0[a] = 2.0f * g_a * Cj0[a] + g_b * Cj1[a];
+        gj1[a] = 2.0f * g_c * Cj1[a] + g_b * Cj0[a];
+      }
+      float gp[3];
+      gp[0] = gu * cam.fx * iz - gj0[2] * cam.fx * iz2;
+      gp[1] = gv * cam.fy * iz - gj1[2] * cam.fy * iz2;
+      gp[2] = -gu * cam.fx * x * iz2 - gv * cam.fy * y * iz2 - gj0[0] * cam.fx * iz2 +
+              gj0[2] * 2.0f * cam.fx * x * iz2 * iz - gj1[1] * cam.fy * iz2 + gj1[2] * 2.0f * cam.fy * y * iz2 * iz;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) gmean[b] += cam.V[0][b] * gp[0] + cam.V[1][b] * gp[1] + cam.V[2][b] * gp[2];
+      // World covariance gradient V^T G V.
+      float T1[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) T1[a][b] = cam.V[0][a] * G[0][b] + cam.V[1][a] * G[1][b] + cam.V[2][a] * G[2][b];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) Gw[a][b] += T1[a][0] * cam.V[0][b] + T1[a][1] * cam.V[1][b] + T1[a][2] * cam.V[2][b];
+    }
+    if (colour) {
+      float dir[3] = {mean[0] - cam.centre[0], mean[1] - cam.centre[1], mean[2] - cam.centre[2]};
+      float len = sqrtf(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+      if (len > 0.0f) {
+        float il = 1.0f / len;
+        float d[3] = {dir[0] * il, dir[1] * il, dir[2] * il};
+        float Y[16], GY[16][3];
+        sh_basis(deg, d, Y);
+        sh_basis_grad(deg, d, GY);
+        const int K = (deg + 1) * (deg + 1);
+        float gd[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          gsh[c] += gr[c];
+#pragma unroll
+          for (int k = 1; k < 16; ++k) {
+            if (k >= K) continue;
+            gsh[k * 3 + c] += Y[k] * gr[c];
+            float w = gr[c] * shc[k * 3 + c];
+            gd[0] += w * GY[k][0];
+            gd[1] += w * GY[k][1];
+            gd[2] += w * GY[k][2];
+          }
+        }
+        float dd = d[0] * gd[0] + d[1] * gd[1] + d[2] * gd[2];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) gmean[e] += (gd[e] - d[e] * dd) * il;
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gsh[c] += gr[c];
+      }
+    }
+  }
+  // Sigma = M M^T, M = R diag(s): dL/dM = 2 Gw M; log-scale and rotation.
+  const float s[3] = {expf(q[3]), expf(q[4]), expf(q[5])};
+  float R[3][3];
+  const float qlen = quat_to_mat(q + 6, R);
+  float gR[3][3], gls[3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float gM = 2.0f * (Gw[a][0] * R[0][b] * s[b] + Gw[a][1] * R[1][b] * s[b] + Gw[a][2] * R[2][b] * s[b]);
+      gR[a][b] = gM * s[b];
+      acc += gM * R[a][b];
+    }
+    gls[b] = acc * s[b];
+  }
+  const float w = q[6] / qlen, qx = q[7] / qlen, qy = q[8] / qlen, qz = q[9] / qlen;
+  float gn[4];
+  gn[0] = 2.0f * (-qz * gR[0][1] + qy * gR[0][2] + qz * gR[1][0] - qx * gR[1][2] - qy * gR[2][0] + qx * gR[2][1]);
+  gn[1] = 2.0f * (qy * gR[0][1] + qz * gR[0][2] + qy * gR[1][0] - 2.0f * qx * gR[1][1] - w * gR[1][2] +
+                  qz * gR[2][0] + w * gR[2][1] - 2.0f * qx * gR[2][2]);
+  gn[2] = 2.0f * (-2.0f * qy * gR[0][0] + qx * gR[0][1] + w * gR[0][2] + qx * gR[1][0] + qz * gR[1][2] -
+                  w * gR[2][0] + qz * gR[2][1] - 2.0f * qy * gR[2][2]);
+  gn[3] = 2.0f * (-2.0f * qz * gR[0][0] - w * gR[0][1] + qx * gR[0][2] + w * gR[1][0] - 2.0f * qz * gR[1][1] +
+                  qy * gR[1][2] + qx * gR[2][0] + qy * gR[2][1]);
+  const float nq[4] = {w, qx, qy, qz};
+  const float dotn = nq[0] * gn[0] + nq[1] * gn[1] + nq[2] * gn[2] + nq[3] * gn[3];
+  const float rawlen = sqrtf(q[6] * q[6] + q[7] * q[7] + q[8] * q[8] + q[9] * q[9]);
+  float gq[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) gq[k] = (rawlen > 1e-12f ? gn[k] - nq[k] * dotn : gn[k]) / qlen;
+  float* o = shape + size_t(i) * 10;
+  o[0] = gmean[0]; o[1] = gmean[1]; o[2] = gmean[2];
+  o[3] = gls[0]; o[4] = gls[1]; o[5] = gls[2];
+  o[6] = gq[0]; o[7] = gq[1]; o[8] = gq[2]; o[9] = gq[3];
+  if (deg > 0 && sh_grad) {
+    float4* so = reinterpret_cast<float4*>(sh_grad + size_t(i) * 48);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) so[k] = make_float4(gsh[4 * k], gsh[4 * k + 1], gsh[4 * k + 2], gsh[4 * k + 3]);
+  }
+  if (grads_out) {
+    float4* go = reinterpret_cast<float4*>(grads_out + size_t(i) * 16);
+    go[0] = make_float4(gcol[0], gcol[1], gcol[2], gop);
+  }
+}
+
+// ADAM.main (gpu.js:422-443): SLOT maps the 14 trainable numbers (gpu.js:420).
+__global__ __launch_bounds__(256) void adam_kernel(gsr_step st, float* __restrict__ gs,
+                                                   const float* __restrict__ grads, const float* __restrict__ shape,
+                                                   float* __restrict__ moments) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.count) return;
+  const int slot[14] = {12, 13, 14, 7, 0, 1, 2, 4, 5, 6, 8, 9, 10, 11};
+  const float bc1 = 1.0f - powf(0.9f, st.t);
+  const float bc2 = 1.0f - powf(0.999f, st.t);
+  float* b = gs + size_t(i) * 16;
+  float* m = moments + size_t(i) * 28;
+#pragma unroll
+  for (int k = 0; k < 14; ++k) {
+    float g = k < 4 ? grads[size_t(i) * 16 + k] : shape[size_t(i) * 10 + k - 4];
+    float lr = st.lr_rotation;
+    if (k < 3) lr = st.lr_color;
+    else if (k == 3) lr = st.lr_opacity;
+    else if (k < 7) lr = st.lr_mean;
+    else if (k < 10) lr = st.lr_scale;
+    float mk = 0.9f * m[k] + 0.1f * g;
+    float vk = 0.999f * m[14 + k] + 0.001f * g * g;
+    m[k] = mk;
+    m[14 + k] = vk;
+    float update = lr * (mk / bc1) / (sqrtf(vk / bc2) + 1e-8f);
+    b[slot[k]] -= update;
+  }
+#pragma unroll
+  for (int c = 12; c < 15; ++c) b[c] = fminf(1.0f, fmaxf(0.0f, b[c]));
+  float qv[4] = {b[8], b[9], b[10], b[11]};
+  float len = fmaxf(sqrtf(qv[0] * qv[0] + qv[1] * qv[1] + qv[2] * qv[2] + qv[3] * qv[3]), 1e-12f);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) b[8 + k] = qv[k] / len;
+}
+
+// STATS.main (gpu.js:491-497).
+__global__ __launch_bounds__(256) void stats_kernel(uint32_t n, const float* __restrict__ grads,
+                                                    const float* __restrict__ splats, float* __restrict__ st) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || splats[size_t(i) * 12 + 11] < 0.5f) return;
+  float gu = grads[size_t(i) * 16 + 4], gv = grads[size_t(i) * 16 + 5];
+  st[size_t(i) * 2] += sqrtf(gu * gu + gv * gv);
+  st[size_t(i) * 2 + 1] += 1.0f;
+}
+
+}  // namespace
+
+void launch_shape(hipStream_t s, const gsr_camera* cams, uint32_t num_views, const float* gs, const float* sh,
+                  int sh_degree, uint32_t n, float* grads, uint64_t grads_view_stride, float* shape,
+                  float* sh_grad, float* grads_out, int consume) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(shape_kernel, dim3((n + 255) / 256), dim3(256), 0, s, cams, num_views, gs, sh, sh_degree, n,
+                     grads, grads_view_stride, shape, sh_grad, grads_out, consume);
+}
+
+void launch_adam(hipStream_t s, const gsr_step& step, float* gs, const float* grads, const float* shape,
+                 float* moments) {
+  if (step.count == 0) return;
+  hipLaunchKernelGGL(adam_kernel, dim3((step.count + 255) / 256), dim3(256), 0, s, step, gs, grads, shape, moments);
+}
+
+void launch_stats(hipStream_t s, const gsr_step& step, const float* grads, const float* splats, float* stats) {
+  if (step.count == 0) return;
+  hipLaunchKernelGGL(stats_kernel, dim3((step.count + 255) / 256), dim3(256), 0, s, step.count, grads, splats,
+                     stats);
+}
+
+}  // namespace gsr

@@ -1,0 +1,467 @@
+// Tile blending: FORWARD (trainer/gpu.js:105-137) and the backward of BACKWARD_SG
+// (gpu.js:204-293), plus the loss gradient (SSIM_COEF / LOSS_GRAD, gpu.js:336-409).
+//
+// One wave64 owns one 16x16 tile; each lane owns four pixels (column lane & 15,
+// rows (lane >> 4) + 4 i).  The tile's list is staged 64 entries at a time in
+// LDS (each lane gathers one 48-byte splat record; the next batch is prefetched
+// into registers while the current one is blended), every lane then reads the
+// batch back as broadcasts.  A wave stops as soon as all of its 256 pixels have
+// T <= 1e-4, so saturated tiles retire early.
+//
+// The backward walks the same list front to back and recomputes transmittance
+// with exactly the forward's arithmetic, so it stops at the same entry for each
+// pixel.  The colour behind an entry is C_final - (accumulated colour), C_final
+// being the forward's image.  Per entry, the four pixels of a lane are summed in
+// registers, the wave reduces with DPP, and nine lanes issue one float atomic
+// each into the 16-float gradient record.
+#include "gsr_internal.h"
+
+namespace gsr {
+
+namespace {
+
+constexpr int kPix = 4;  // pixels per lane
+
+struct Tile {
+  uint32_t view;
+  uint32_t x0, y0;      // pixel origin
+  uint32_t width, height;
+  uint32_t start, end;  // entry range (view-local)
+  const uint32_t* order;
+  const float* splats;
+  uint64_t pixel_base;
+  uint32_t gtile;       // global tile id (ranges index)
+};
+
+__device__ inline Tile load_tile(const RasterArgs& a, uint32_t w) {
+  TileWork tw = a.work[w];
+  ViewDesc vd = a.views[tw.view];
+  Tile t;
+  t.view = tw.view;
+  uint32_t tx = tw.xy & 0xFFFFu, ty = tw.xy >> 16;
+  t.x0 = tx * kTile;
+  t.y0 = ty * kTile;
+  t.width = vd.width;
+  t.height = vd.height;
+  t.gtile = vd.tile_base + ty * vd.tiles_x + tx;
+  t.start = a.ranges[size_t(t.gtile) * 2];
+  t.end = a.ranges[size_t(t.gtile) * 2 + 1];
+  t.order = a.order + vd.entry_base;
+  t.splats = a.splats + vd.splat_base * 12;
+  t.pixel_base = vd.pixel_base;
+  return t;
+}
+
+// Records of entries [k0, k0 + 64): lane j fetches entry k0 + j.
+struct Batch {
+  float4 r0, r1, r2;  // (u, v, A, B), (C, opacity, radius, depth), (r, g, b, valid)
+  uint32_t gi;
+};
+
+__device__ inline Batch fetch_batch(const Tile& t, uint32_t k0, uint32_t lane) {
+  Batch b;
+  uint32_t k = k0 + lane;
+  if (k < t.end) {
+    b.gi = t.order[k];
+    const float4* r = reinterpret_cast<const float4*>(t.splats + size_t(b.gi) * 12);
+    b.r0 = r[0];
+    b.r1 = r[1];
+    b.r2 = r[2];
+  } else {
+    b.gi = 0;
+    b.r0 = b.r1 = make_float4(0.f, 0.f, 0.f, 0.f);
+    b.r2 = make_float4(0.f, 0.f, 0.f, 0.f);  // valid = 0
+  }
+  return b;
+}
+
+// alphaAt (gpu.js:92-103).  Returns alpha, or 0 for a miss.
+__device__ inline float alpha_at(float dx, float dy, float A, float B, float C, float op, float rad) {
+  if (fabsf(dx) > rad || fabsf(dy) > rad) return 0.0f;
+  float power = -0.5f * (A * dx * dx + C * dy * dy) - B * dx * dy;
+  if (power > 0.0f) return 0.0f;
+  float alpha = fminf(kAlphaMax, op * __expf(power));
+  if (alpha < kAlphaMin) return 0.0f;
+  return alpha;
+}
+
+__device__ inline bool any_active(const float T[kPix]) {
+  bool act = false;
+#pragma unroll
+  for (int i = 0; i < kPix; ++i) act |= T[i] > kTStop;
+  return __ballot(act) != 0;
+}
+
+// ---- forward -------------------------------------------------------------------
+
+__device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) {
+  const uint32_t lane = __lane_id();
+  const Tile t = load_tile(a, w);
+  const uint32_t x = t.x0 + (lane & 15);
+  const float px = float(x) + 0.5f;
+  float py[kPix], T[kPix], cr[kPix], cg[kPix], cb[kPix];
+  uint32_t last[kPix];
+#pragma unroll
+  for (int i = 0; i < kPix; ++i) {
+    uint32_t y = t.y0 + (lane >> 4) + 4 * i;
+    py[i] = float(y) + 0.5f;
+    bool inside = x < t.width && y < t.height;
+    T[i] = inside ? 1.0f : 0.0f;  // outside pixels count as done
+    cr[i] = cg[i] = cb[i] = 0.0f;
+    last[i] = t.start;
+  }
+  Batch nb = fetch_batch(t, t.start, lane);
+  for (uint32_t k0 = t.start; k0 < t.end; k0 += 64) {
+    if (!any_active(T)) break;
+    __syncthreads();
+    lds[lane][0] = nb.r0;
+    lds[lane][1] = nb.r1;
+    lds[lane][2] = nb.r2;
+    __syncthreads();
+    if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
+    const uint32_t nk = min(64u, t.end - k0);
+    for (uint32_t j = 0; j < nk; ++j) {
+      if ((j & 7) == 0 && j != 0 && !any_active(T)) break;
+      const float4 r0 = lds[j][0], r1 = lds[j][1], r2 = lds[j][2];
+      if (r2.w < 0.5f) continue;
+#pragma unroll
+      for (int i = 0; i < kPix; ++i) {
+        if (!(T[i] > kTStop)) continue;
+        float dx = px - r0.x, dy = py[i] - r0.y;
+        float alpha = alpha_at(dx, dy, r0.z, r0.w, r1.x, r1.y, r1.z);
+        if (alpha == 0.0f) continue;
+        float wgt = alpha * T[i];
+        cr[i] = fmaf(r2.x, wgt, cr[i]);
+        cg[i] = fmaf(r2.y, wgt, cg[i]);
+        cb[i] = fmaf(r2.z, wgt, cb[i]);
+        T[i] = T[i] * (1.0f - alpha);
+        last[i] = k0 + j + 1;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kPix; ++i) {
+    uint32_t y = t.y0 + (lane >> 4) + 4 * i;
+    if (x >= t.width || y >= t.height) continue;
+    size_t pix = t.pixel_base + size_t(y) * t.width + x;
+    a.image[pix * 3] = cr[i];
+    a.image[pix * 3 + 1] = cg[i];
+    a.image[pix * 3 + 2] = cb[i];
+    if (a.state) {
+      a.state[pix * 2] = T[i];
+      a.state[pix * 2 + 1] = float(last[i]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void forward_kernel(RasterArgs a) {
+  __shared__ float4 lds[64][3];
+  forward_tile(a, blockIdx.x, lds);
+}
+
+__device__ inline uint32_t pop_work(uint32_t* queue, uint32_t n) {
+  uint32_t w0 = 0;
+  if (__lane_id() == 0) w0 = atomicAdd(queue, n);
+  return __builtin_amdgcn_readfirstlane(w0);
+}
+
+__global__ __launch_bounds__(64) void forward_persistent_kernel(RasterArgs a) {
+  __shared__ float4 lds[64][3];
+  for (;;) {
+    uint32_t w0 = pop_work(a.queue, a.tile_batch);
+    if (w0 >= a.num_work) break;
+    uint32_t w1 = min(a.num_work, w0 + a.tile_batch);
+    for (uint32_t w = w0; w < w1; ++w) forward_tile(a, w, lds);
+  }
+}
+
+// ---- backward ------------------------------------------------------------------
+
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ inline float dpp_src(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+
+// Sum over the wave; the total lands in lane 63.
+__device__ inline float wave_sum63(float v) {
+  v += dpp_src<0xB1>(v);        // quad_perm [1,0,3,2]
+  v += dpp_src<0x4E>(v);        // quad_perm [2,3,0,1]
+  v += dpp_src<0x124>(v);       // row_ror:4
+  v += dpp_src<0x128>(v);       // row_ror:8  -> every lane holds its row's sum
+  v += dpp_src<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_src<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
+__device__ inline float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+
+__device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3], uint32_t* gi_lds) {
+  const uint32_t lane = __lane_id();
+  const Tile t = load_tile(a, w);
+  const uint32_t x = t.x0 + (lane & 15);
+  const float px = float(x) + 0.5f;
+  float py[kPix], T[kPix], ar[kPix], ag[kPix], ab[kPix];
+  float fr[kPix], fg[kPix], fb[kPix], dr[kPix], dg[kPix], db[kPix];
+#pragma unroll
+  for (int i = 0; i < kPix; ++i) {
+    uint32_t y = t.y0 + (lane >> 4) + 4 * i;
+    py[i] = float(y) + 0.5f;
+    bool inside = x < t.width && y < t.height;
+    T[i] = inside ? 1.0f : 0.0f;
+    ar[i] = ag[i] = ab[i] = 0.0f;
+    fr[i] = fg[i] = fb[i] = dr[i] = dg[i] = db[i] = 0.0f;
+    if (inside) {
+      size_t pix = t.pixel_base + size_t(y) * t.width + x;
+      fr[i] = a.image[pix * 3];
+      fg[i] = a.image[pix * 3 + 1];
+      fb[i] = a.image[pix * 3 + 2];
+      dr[i] = a.dl[pix * 3];
+      dg[i] = a.dl[pix * 3 + 1];
+      db[i] = a.dl[pix * 3 + 2];
+    }
+  }
+  float* grads = a.grads + size_t(t.view) * a.grads_view_stride;
+  Batch nb = fetch_batch(t, t.start, lane);
+  for (uint32_t k0 = t.start; k0 < t.end; k0 += 64) {
+    if (!any_active(T)) break;
+    __syncthreads();
+    lds[lane][0] = nb.r0;
+    lds[lane][1] = nb.r1;
+    lds[lane][2] = nb.r2;
+    gi_lds[lane] = nb.gi;
+    __syncthreads();
+    if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
+    const uint32_t nk = min(64u, t.end - k0);
+    for (uint32_t j = 0; j < nk; ++j) {
+      if ((j & 7) == 0 && j != 0 && !any_active(T)) break;
+      const float4 r0 = lds[j][0], r1 = lds[j][1], r2 = lds[j][2];
+      if (r2.w < 0.5f) continue;
+      const float A = r0.z, B = r0.w, C = r1.x, op = r1.y;
+      float g[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) g[q] = 0.0f;
+      bool hit = false;
+#pragma unroll
+      for (int i = 0; i < kPix; ++i) {
+        if (!(T[i] > kTStop)) continue;
+        float dx = px - r0.x, dy = py[i] - r0.y;
+        float alpha = alpha_at(dx, dy, A, B, C, op, r1.z);
+        if (alpha == 0.0f) continue;
+        hit = true;
+        // Same arithmetic as forward_tile: acc and T follow it bit for bit.
+        float wgt = alpha * T[i];
+        ar[i] = fmaf(r2.x, wgt, ar[i]);
+        ag[i] = fmaf(r2.y, wgt, ag[i]);
+        ab[i] = fmaf(r2.z, wgt, ab[i]);
+        const float Tk = T[i];
+        const float oma = 1.0f - alpha;
+        T[i] = Tk * oma;
+        // gpu.js:266-268: colour gradient and dLoss/dalpha; behind = C_final - acc.
+        g[0] += dr[i] * wgt;
+        g[1] += dg[i] * wgt;
+        g[2] += db[i] * wgt;
+        float inv = 1.0f / oma;
+        float dA = dr[i] * (Tk * r2.x - (fr[i] - ar[i]) * inv) + dg[i] * (Tk * r2.y - (fg[i] - ag[i]) * inv) +
+                   db[i] * (Tk * r2.z - (fb[i] - ab[i]) * inv);
+        if (alpha < kAlphaMax) {
+          // gpu.js:270-281: opacity logit and screen (u, v, conic) gradients.
+          g[3] += dA * alpha * (1.0f - op);
+          float dP = dA * alpha;
+          g[4] += dP * (A * dx + B * dy);
+          g[5] += dP * (C * dy + B * dx);
+          g[6] += dP * -0.5f * dx * dx;
+          g[7] += dP * -dx * dy;
+          g[8] += dP * -0.5f * dy * dy;
+        }
+      }
+      if (__ballot(hit) == 0) continue;
+      float s[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) s[q] = lane63(wave_sum63(g[q]));
+      float val = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) val = lane == uint32_t(q) ? s[q] : val;
+      if (lane < 9 && val != 0.0f) {
+        const uint32_t gi = gi_lds[j];
+        unsafeAtomicAdd(grads + size_t(gi) * 16 + lane, val);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void backward_kernel(RasterArgs a) {
+  __shared__ float4 lds[64][3];
+  __shared__ uint32_t gi_lds[64];
+  backward_tile(a, blockIdx.x, lds, gi_lds);
+}
+
+__global__ __launch_bounds__(64) void backward_persistent_kernel(RasterArgs a) {
+  __shared__ float4 lds[64][3];
+  __shared__ uint32_t gi_lds[64];
+  for (;;) {
+    uint32_t w0 = pop_work(a.queue, a.tile_batch);
+    if (w0 >= a.num_work) break;
+    uint32_t w1 = min(a.num_work, w0 + a.tile_batch);
+    for (uint32_t w = w0; w < w1; ++w) backward_tile(a, w, lds, gi_lds);
+  }
+}
+
+// ---- loss gradient (gpu.js:336-409) ---------------------------------------------
+
+__global__ __launch_bounds__(256) void ssim_coef_kernel(const ViewDesc* __restrict__ views,
+                                                        const float* __restrict__ image,
+                                                        const float* __restrict__ target, float* __restrict__ coef) {
+  const ViewDesc vd = views[blockIdx.y];
+  const uint32_t P = vd.width * vd.height;
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const int W = int(vd.width), H = int(vd.height);
+  const int x = int(p % vd.width), y = int(p / vd.width);
+  const bool inside = x >= 3 && y >= 3 && x + 3 < W && y + 3 < H;
+  const float* img = image + vd.pixel_base * 3;
+  const float* tgt = target + vd.pixel_base * 3;
+  float* out = coef + (vd.pixel_base + p) * 9;
+  for (int c = 0; c < 3; ++c) {
+    float ca = 0.0f, cb = 0.0f, cd = 0.0f;
+    if (inside) {
+      float mx = 0.0f, my = 0.0f, sxx = 0.0f, syy = 0.0f, sxy = 0.0f;
+      for (int dy = 0; dy < 7; ++dy)
+        for (int dx = 0; dx < 7; ++dx) {
+          size_t q = (size_t(y + dy - 3) * W + x + dx - 3) * 3 + c;
+          float xv = img[q], yv = tgt[q];
+          mx += xv; my += yv; sxx += xv * xv; syy += yv * yv; sxy += xv * yv;
+        }
+      mx /= 49.0f; my /= 49.0f; sxx /= 49.0f; syy /= 49.0f; sxy /= 49.0f;
+      const float c1 = 0.0001f, c2 = 0.0009f;
+      float a1 = 2.0f * mx * my + c1;
+      float a2 = 2.0f * (sxy - mx * my) + c2;
+      float b1 = mx * mx + my * my + c1;
+      float b2 = (sxx - mx * mx) + (syy - my * my) + c2;
+      float ssim = a1 * a2 / (b1 * b2);
+      ca = ssim * (2.0f * my / a1 - 2.0f * my / a2 - 2.0f * mx / b1 + 2.0f * mx / b2);
+      cb = -ssim / b2;
+      cd = 2.0f * ssim / a2;
+    }
+    out[c * 3] = ca;
+    out[c * 3 + 1] = cb;
+    out[c * 3 + 2] = cd;
+  }
+}
+
+__device__ inline float sign_f(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
+
+__global__ __launch_bounds__(256) void loss_grad_kernel(const ViewDesc* __restrict__ views,
+                                                        const float* __restrict__ image,
+                                                        const float* __restrict__ target,
+                                                        const float* __restrict__ coef, float lambda,
+                                                        float* __restrict__ grad) {
+  const ViewDesc vd = views[blockIdx.y];
+  const uint32_t P = vd.width * vd.height;
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const int W = int(vd.width), H = int(vd.height);
+  const int x0 = int(p % vd.width), y0 = int(p / vd.width);
+  const float n = float(P * 3u);
+  const float windows = float(max(0, W - 6) * max(0, H - 6) * 3);
+  const size_t pix = vd.pixel_base + p;
+  for (int c = 0; c < 3; ++c) {
+    float xv = image[pix * 3 + c], yv = target[pix * 3 + c];
+    float g = (1.0f - lambda) * sign_f(xv - yv) / n;
+    if (lambda > 0.0f && windows > 0.0f) {
+      float sa = 0.0f, sb = 0.0f, sd = 0.0f;
+      for (int dy = -3; dy <= 3; ++dy)
+        for (int dx = -3; dx <= 3; ++dx) {
+          int cx = x0 + dx, cy = y0 + dy;
+          if (cx < 0 || cy < 0 || cx >= W || cy >= H) continue;
+          const float* k = coef + (vd.pixel_base + size_t(cy) * W + cx) * 9 + c * 3;
+          sa += k[0]; sb += k[1]; sd += k[2];
+        }
+      float dS = (sa + 2.0f * xv * sb + yv * sd) / 49.0f / windows;
+      g -= lambda * dS;
+    }
+    grad[pix * 3 + c] = g;
+  }
+}
+
+__global__ __launch_bounds__(256) void zero_kernel(uint4* __restrict__ p, size_t n16, uint32_t* __restrict__ tail,
+                                                   uint32_t ntail) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (; i < n16; i += stride) p[i] = make_uint4(0u, 0u, 0u, 0u);
+  if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(256) void fill_work_kernel(const ViewDesc* __restrict__ views, uint32_t num_views,
+                                                        uint32_t total_tiles, TileWork* __restrict__ work) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total_tiles) return;
+  uint32_t v = 0;
+  while (v + 1 < num_views && views[v + 1].tile_base <= g) ++v;
+  const ViewDesc vd = views[v];
+  const uint32_t t = g - vd.tile_base;
+  TileWork w;
+  w.view = v;
+  w.xy = (t % vd.tiles_x) | ((t / vd.tiles_x) << 16);
+  work[g] = w;
+}
+
+}  // namespace
+
+void launch_fill_work(hipStream_t s, const ViewDesc* views, uint32_t num_views, uint32_t total_tiles,
+                      TileWork* work) {
+  if (total_tiles == 0) return;
+  hipLaunchKernelGGL(fill_work_kernel, dim3((total_tiles + 255) / 256), dim3(256), 0, s, views, num_views,
+                     total_tiles, work);
+}
+
+void launch_forward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid) {
+  if (a.num_work == 0) return;
+  if (persistent)
+    hipLaunchKernelGGL(forward_persistent_kernel, dim3(grid), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL(forward_kernel, dim3(a.num_work), dim3(64), 0, s, a);
+}
+
+void launch_backward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid) {
+  if (a.num_work == 0) return;
+  if (persistent)
+    hipLaunchKernelGGL(backward_persistent_kernel, dim3(grid), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL(backward_kernel, dim3(a.num_work), dim3(64), 0, s, a);
+}
+
+void raster_occupancy(int* forward_per_cu, int* backward_per_cu) {
+  int f = 0, b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&f, forward_persistent_kernel, 64, 0) != hipSuccess) f = 8;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, backward_persistent_kernel, 64, 0) != hipSuccess) b = 8;
+  *forward_per_cu = f;
+  *backward_per_cu = b;
+}
+
+void launch_loss_grad(hipStream_t s, const ViewDesc* views, const ViewDesc* host_views, uint32_t num_views,
+                      const float* image, const float* target, float lambda, float* coef, float* grad) {
+  uint32_t maxp = 0;
+  for (uint32_t v = 0; v < num_views; ++v) {
+    uint32_t p = host_views[v].width * host_views[v].height;
+    maxp = p > maxp ? p : maxp;
+  }
+  if (maxp == 0) return;
+  dim3 grid((maxp + 255) / 256, num_views);
+  if (lambda > 0.0f) hipLaunchKernelGGL(ssim_coef_kernel, grid, dim3(256), 0, s, views, image, target, coef);
+  hipLaunchKernelGGL(loss_grad_kernel, grid, dim3(256), 0, s, views, image, target, coef, lambda, grad);
+}
+
+void launch_zero(hipStream_t s, void* p, size_t bytes) {
+  if (bytes == 0) return;
+  // Device buffers from gsr_alloc are 256-byte aligned; bytes is a multiple of 4.
+  size_t n16 = bytes / 16;
+  uint32_t ntail = uint32_t((bytes % 16) / 4);
+  uint32_t* tail = reinterpret_cast<uint32_t*>(static_cast<char*>(p) + n16 * 16);
+  size_t blocks = (n16 + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(zero_kernel, dim3(uint32_t(blocks)), dim3(256), 0, s, static_cast<uint4*>(p), n16, tail,
+                     ntail);
+}
+
+}  // namespace gsr

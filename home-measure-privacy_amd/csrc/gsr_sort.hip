@@ -1,0 +1,468 @@
+// Tile binning with the ordering contract of tileLists (trainer/gpu.js:450-481):
+// per tile, splats in (depth, Gaussian index) order, tiles in row-major order.
+//
+// The order is a stable LSD radix sort of the 64-bit keys (tile << 32 | depth
+// bits) over the entries, split where it pays on this chip:
+//   * the 32 depth bits are sorted once per Gaussian (N keys, four 8-bit passes,
+//     stable, so equal depths keep index order), because every entry of a
+//     Gaussian shares its depth;
+//   * the tile digit is one counting pass over the entries: blocks walk a chunk of
+//     the depth-sorted Gaussians in order and keep one counter per tile in LDS,
+//     so each entry is written exactly once (no multi-pass radix over E).
+// Both sorts are segmented by view (one segment per camera of a batch).
+#include "gsr_internal.h"
+
+namespace gsr {
+
+namespace {
+
+constexpr int kRadixBits = 8;
+constexpr int kBins = 1 << kRadixBits;
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;                            // per thread
+constexpr int kSortChunk = kSortThreads * kSortItems;     // 4096 keys per block
+constexpr int kWaveItems = kSortChunk / (kSortThreads / 64);  // 1024 per wave
+
+__device__ inline uint64_t lanemask_lt() {
+  uint32_t lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Lanes of the wave whose 8-bit digit equals this lane's (active lanes only).
+__device__ inline uint64_t match_digit(uint32_t d, bool active) {
+  uint64_t m = __ballot(active);
+#pragma unroll
+  for (int b = 0; b < kRadixBits; ++b) {
+    bool bit = (d >> b) & 1u;
+    uint64_t bb = __ballot(bit);
+    m &= bit ? bb : ~bb;
+  }
+  return m;
+}
+
+// Digit histogram of one 4096-key chunk of one view: hist[v][digit][chunk].
+__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t* __restrict__ keys, uint32_t n,
+                                                                   uint32_t chunks, int shift,
+                                                                   uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[kBins];
+  const uint32_t v = blockIdx.y, c = blockIdx.x;
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const size_t base = size_t(v) * n + size_t(c) * kSortChunk;
+  const uint32_t len = min(uint32_t(kSortChunk), n - c * kSortChunk);
+  for (uint32_t j0 = 0; j0 < uint32_t(kSortChunk); j0 += kSortThreads) {
+    uint32_t j = j0 + threadIdx.x;
+    bool active = j < len;
+    uint32_t d = active ? (keys[base + j] >> shift) & (kBins - 1) : 0u;
+    uint64_t peers = match_digit(d, active);
+    if (active && (peers & lanemask_lt()) == 0) atomicAdd(&h[d], uint32_t(__popcll(peers)));
+  }
+  __syncthreads();
+  hist[(size_t(v) * kBins + threadIdx.x) * chunks + c] = h[threadIdx.x];
+}
+
+// Stable scatter of one chunk: wave w owns keys [w*1024, (w+1)*1024) of the chunk
+// and ranks them in 16 rounds of 64 with ballot peer masks.
+__global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
+    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t n, uint32_t chunks,
+    int shift, const uint32_t* __restrict__ hist, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out) {
+  __shared__ uint32_t wcnt[4][kBins];
+  __shared__ uint32_t gpos[4][kBins];
+  const uint32_t v = blockIdx.y, c = blockIdx.x;
+  const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) wcnt[k][threadIdx.x] = 0;
+  __syncthreads();
+  const size_t seg = size_t(v) * n;
+  const uint32_t first = c * kSortChunk + w * kWaveItems;
+  uint32_t kr[kSortItems], vr[kSortItems], dl[kSortItems];
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    uint32_t j = first + r * 64 + lane;
+    bool active = j < n;
+    uint32_t key = active ? keys_in[seg + j] : 0u;
+    uint32_t val = active ? (vals_in ? vals_in[seg + j] : j) : 0u;
+    uint32_t d = (key >> shift) & (kBins - 1);
+    uint64_t peers = match_digit(d, active);
+    uint32_t rank = __popcll(peers & lt);
+    uint32_t before = active ? wcnt[w][d] : 0u;
+    if (active && rank == 0) wcnt[w][d] = before + uint32_t(__popcll(peers));
+    kr[r] = key;
+    vr[r] = val;
+    dl[r] = active ? (d << 16) | (before + rank) : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  {
+    const uint32_t d = threadIdx.x;
+    uint32_t g = hist[(size_t(v) * kBins + d) * chunks + c];
+    uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d];
+    gpos[0][d] = g;
+    gpos[1][d] = g + c0;
+    gpos[2][d] = g + c0 + c1;
+    gpos[3][d] = g + c0 + c1 + c2;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    if (dl[r] == 0xFFFFFFFFu) continue;
+    uint32_t d = dl[r] >> 16, local = dl[r] & 0xFFFFu;
+    uint32_t dst = gpos[w][d] + local;
+    keys_out[seg + dst] = kr[r];
+    vals_out[seg + dst] = vr[r];
+  }
+}
+
+// ---- segmented exclusive scan over equal-length segments (u32) ------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanPiece = kScanThreads * kScanItems;
+
+__device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* tmp, uint32_t& total) {
+  // Inclusive scan of x over the block through LDS (Hillis-Steele), returns exclusive.
+  const int t = threadIdx.x;
+  tmp[t] = x;
+  __syncthreads();
+  for (int off = 1; off < int(blockDim.x); off <<= 1) {
+    uint32_t y = t >= off ? tmp[t - off] : 0u;
+    __syncthreads();
+    tmp[t] += y;
+    __syncthreads();
+  }
+  uint32_t incl = tmp[t];
+  total = tmp[blockDim.x - 1];
+  __syncthreads();
+  return incl - x;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* __restrict__ data, size_t seg_len,
+                                                                   uint32_t pieces, uint32_t* __restrict__ piece) {
+  __shared__ uint32_t tmp[kScanThreads];
+  const uint32_t s = blockIdx.y, p = blockIdx.x;
+  const size_t base = size_t(s) * seg_len + size_t(p) * kScanPiece;
+  const size_t lim = seg_len - size_t(p) * kScanPiece;
+  uint32_t sum = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    size_t j = size_t(k) * kScanThreads + threadIdx.x;
+    if (j < lim) sum += data[base + j];
+  }
+  uint32_t total;
+  block_exclusive_scan(sum, tmp, total);
+  if (threadIdx.x == 0) piece[size_t(s) * pieces + p] = total;
+}
+
+__global__ __launch_bounds__(1024) void scan_pieces_kernel(uint32_t* __restrict__ piece, uint32_t pieces) {
+  __shared__ uint32_t tmp[1024];
+  const uint32_t s = blockIdx.x;
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < pieces; b += 1024) {
+    uint32_t j = b + threadIdx.x;
+    uint32_t x = j < pieces ? piece[size_t(s) * pieces + j] : 0u;
+    uint32_t total;
+    uint32_t ex = block_exclusive_scan(x, tmp, total);
+    if (j < pieces) piece[size_t(s) * pieces + j] = carry + ex;
+    carry += total;
+  }
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(uint32_t* __restrict__ data, size_t seg_len,
+                                                                  uint32_t pieces,
+                                                                  const uint32_t* __restrict__ piece) {
+  __shared__ uint32_t tmp[kScanThreads];
+  const uint32_t s = blockIdx.y, p = blockIdx.x;
+  const size_t base = size_t(s) * seg_len + size_t(p) * kScanPiece;
+  const size_t lim = seg_len - size_t(p) * kScanPiece;
+  // Thread t owns kScanItems consecutive elements.
+  uint32_t x[kScanItems];
+  uint32_t sum = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    size_t j = size_t(threadIdx.x) * kScanItems + k;
+    x[k] = j < lim ? data[base + j] : 0u;
+    sum += x[k];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(sum, tmp, total) + piece[size_t(s) * pieces + p];
+  for (int k = 0; k < kScanItems; ++k) {
+    size_t j = size_t(threadIdx.x) * kScanItems + k;
+    if (j < lim) data[base + j] = run;
+    run += x[k];
+  }
+}
+
+void scan_segments(hipStream_t st, uint32_t* data, size_t seg_len, uint32_t segs, uint32_t* piece) {
+  uint32_t pieces = uint32_t((seg_len + kScanPiece - 1) / kScanPiece);
+  dim3 grid(pieces, segs);
+  hipLaunchKernelGGL(scan_reduce_kernel, grid, dim3(kScanThreads), 0, st, data, seg_len, pieces, piece);
+  hipLaunchKernelGGL(scan_pieces_kernel, dim3(segs), dim3(1024), 0, st, piece, pieces);
+  hipLaunchKernelGGL(scan_apply_kernel, grid, dim3(kScanThreads), 0, st, data, seg_len, pieces, piece);
+}
+
+// ---- binning -------------------------------------------------------------------
+// Tile counters live in LDS, one band of at most kBand tiles at a time.
+constexpr uint32_t kBand = 8192;
+
+// Tile index of the t-th tile of a rectangle w tiles wide starting at (x0, y0).
+__device__ inline uint32_t rect_tile(uint32_t t, uint32_t w, float rcp_w, uint32_t x0, uint32_t y0, uint32_t tiles_x) {
+  uint32_t row = uint32_t(float(t) * rcp_w);
+  while (row * w > t) --row;
+  while ((row + 1) * w <= t) ++row;
+  uint32_t col = t - row * w;
+  return (y0 + row) * tiles_x + x0 + col;
+}
+
+struct RectInfo {
+  uint32_t x0, y0, w, cnt;
+  uint32_t gi;
+};
+
+__device__ inline RectInfo rect_info(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                     const uint2* __restrict__ rects, size_t seg, uint32_t pos, uint32_t end) {
+  RectInfo r{0u, 0u, 1u, 0u, 0u};
+  if (pos < end && keys[seg + pos] != kInvalidKey) {
+    r.gi = vals[seg + pos];
+    uint2 rc = rects[seg + r.gi];
+    uint32_t x0 = rc.x & 0xFFFFu, x1 = rc.x >> 16, y0 = rc.y & 0xFFFFu, y1 = rc.y >> 16;
+    if (x0 <= x1 && y0 <= y1) {
+      r.x0 = x0; r.y0 = y0; r.w = x1 - x0 + 1; r.cnt = r.w * (y1 - y0 + 1);
+    }
+  }
+  return r;
+}
+
+// Pass 1: per (chunk, tile) entry counts into matrix[v][chunk][tile].
+// One wave per block walks its chunk's Gaussians in depth order; for each
+// Gaussian the 64 lanes spread over its tiles.
+__global__ __launch_bounds__(64) void bin_count_kernel(const ViewDesc* __restrict__ views, uint32_t n, uint32_t chunk,
+                                                       uint32_t chunks, const uint32_t* __restrict__ keys,
+                                                       const uint32_t* __restrict__ vals,
+                                                       const uint2* __restrict__ rects,
+                                                       const uint64_t* __restrict__ matrix_off,
+                                                       uint32_t* __restrict__ matrix) {
+  extern __shared__ uint32_t cnt[];
+  const uint32_t v = blockIdx.y, c = blockIdx.x, lane = threadIdx.x;
+  const ViewDesc vd = views[v];
+  const uint32_t T = vd.tiles_x * vd.tiles_y;
+  const size_t seg = size_t(v) * n;
+  const uint32_t begin = c * chunk, end = min(n, begin + chunk);
+  uint32_t* out = matrix + matrix_off[v] + size_t(c) * T;
+  for (uint32_t band = 0; band < T; band += kBand) {
+    const uint32_t bw = min(kBand, T - band);
+    for (uint32_t t = lane; t < bw; t += 64) cnt[t] = 0;
+    __syncthreads();
+    for (uint32_t p0 = begin; p0 < end; p0 += 64) {
+      RectInfo ri = rect_info(keys, vals, rects, seg, p0 + lane, end);
+      if (__ballot(ri.cnt != 0) == 0) break;  // sorted: the rest are invalid
+      for (int j = 0; j < 64; ++j) {
+        uint32_t cj = __builtin_amdgcn_readlane(ri.cnt, j);
+        if (cj == 0) continue;
+        uint32_t x0 = __builtin_amdgcn_readlane(ri.x0, j), y0 = __builtin_amdgcn_readlane(ri.y0, j);
+        uint32_t w = __builtin_amdgcn_readlane(ri.w, j);
+        float rcp_w = 1.0f / float(w);
+        for (uint32_t t = lane; t < cj; t += 64) {
+          uint32_t tile = rect_tile(t, w, rcp_w, x0, y0, vd.tiles_x) - band;
+          if (tile < bw) atomicAdd(&cnt[tile], 1u);
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t t = lane; t < bw; t += 64) out[band + t] = cnt[t];
+    __syncthreads();
+  }
+}
+
+// tile_tot[tile_base + t] = sum over chunks of matrix[v][chunk][t].
+__global__ __launch_bounds__(256) void bin_colsum_kernel(const ViewDesc* __restrict__ views, uint32_t chunks,
+                                                         const uint64_t* __restrict__ matrix_off,
+                                                         const uint32_t* __restrict__ matrix,
+                                                         uint32_t* __restrict__ tile_tot) {
+  const uint32_t v = blockIdx.y;
+  const ViewDesc vd = views[v];
+  const uint32_t T = vd.tiles_x * vd.tiles_y;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  const uint32_t* m = matrix + matrix_off[v] + t;
+  uint32_t s = 0;
+  for (uint32_t c = 0; c < chunks; ++c) s += m[size_t(c) * T];
+  tile_tot[vd.tile_base + t] = s;
+}
+
+// Exclusive scan over a view's tiles: ranges (gpu.js:468-474) and the view's
+// entry count.
+__global__ __launch_bounds__(1024) void bin_tilescan_kernel(const ViewDesc* __restrict__ views, uint32_t chunks,
+                                                            const uint32_t* __restrict__ tile_tot,
+                                                            uint32_t* __restrict__ ranges,
+                                                            uint64_t* __restrict__ entries) {
+  __shared__ uint32_t tmp[1024];
+  const uint32_t v = blockIdx.x;
+  const ViewDesc vd = views[v];
+  const uint32_t T = vd.tiles_x * vd.tiles_y;
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < T; b += 1024) {
+    uint32_t t = b + threadIdx.x;
+    uint32_t x = (chunks > 0 && t < T) ? tile_tot[vd.tile_base + t] : 0u;
+    uint32_t total;
+    uint32_t ex = block_exclusive_scan(x, tmp, total);
+    if (t < T) {
+      ranges[size_t(vd.tile_base + t) * 2] = carry + ex;
+      ranges[size_t(vd.tile_base + t) * 2 + 1] = carry + ex + x;
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) entries[v] = carry;
+}
+
+// entry_base of each view = exclusive prefix of the entry counts.
+__global__ void bin_entry_base_kernel(ViewDesc* __restrict__ views, uint32_t num_views,
+                                      const uint64_t* __restrict__ entries) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint64_t run = 0;
+  for (uint32_t v = 0; v < num_views; ++v) {
+    views[v].entry_base = run;
+    run += entries[v];
+  }
+}
+
+// matrix[v][c][t] <- ranges start of t + sum_{c' < c} matrix[v][c'][t].
+__global__ __launch_bounds__(256) void bin_colscan_kernel(const ViewDesc* __restrict__ views, uint32_t chunks,
+                                                          const uint64_t* __restrict__ matrix_off,
+                                                          const uint32_t* __restrict__ ranges,
+                                                          uint32_t* __restrict__ matrix) {
+  const uint32_t v = blockIdx.y;
+  const ViewDesc vd = views[v];
+  const uint32_t T = vd.tiles_x * vd.tiles_y;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  uint32_t* m = matrix + matrix_off[v] + t;
+  uint32_t run = ranges[size_t(vd.tile_base + t) * 2];
+  for (uint32_t c = 0; c < chunks; ++c) {
+    uint32_t x = m[size_t(c) * T];
+    m[size_t(c) * T] = run;
+    run += x;
+  }
+}
+
+// Pass 2: the same walk as bin_count, now handing out slots from per-tile LDS
+// cursors seeded with the scanned matrix.  Gaussians are visited in depth order
+// and a Gaussian's tiles are distinct, so each tile's slots follow (depth, index).
+__global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict__ views, uint32_t n, uint32_t chunk,
+                                                      const uint32_t* __restrict__ keys,
+                                                      const uint32_t* __restrict__ vals,
+                                                      const uint2* __restrict__ rects,
+                                                      const uint64_t* __restrict__ matrix_off,
+                                                      const uint32_t* __restrict__ matrix,
+                                                      uint32_t* __restrict__ order, uint64_t capacity,
+                                                      uint32_t* __restrict__ overflow) {
+  extern __shared__ uint32_t cursor[];
+  const uint32_t v = blockIdx.y, c = blockIdx.x, lane = threadIdx.x;
+  const ViewDesc vd = views[v];
+  const uint32_t T = vd.tiles_x * vd.tiles_y;
+  const size_t seg = size_t(v) * n;
+  const uint32_t begin = c * chunk, end = min(n, begin + chunk);
+  const uint32_t* base = matrix + matrix_off[v] + size_t(c) * T;
+  const uint64_t ebase = vd.entry_base;
+  bool over = false;
+  for (uint32_t band = 0; band < T; band += kBand) {
+    const uint32_t bw = min(kBand, T - band);
+    for (uint32_t t = lane; t < bw; t += 64) cursor[t] = base[band + t];
+    __syncthreads();
+    for (uint32_t p0 = begin; p0 < end; p0 += 64) {
+      RectInfo ri = rect_info(keys, vals, rects, seg, p0 + lane, end);
+      if (__ballot(ri.cnt != 0) == 0) break;
+      for (int j = 0; j < 64; ++j) {
+        uint32_t cj = __builtin_amdgcn_readlane(ri.cnt, j);
+        if (cj == 0) continue;
+        uint32_t x0 = __builtin_amdgcn_readlane(ri.x0, j), y0 = __builtin_amdgcn_readlane(ri.y0, j);
+        uint32_t w = __builtin_amdgcn_readlane(ri.w, j);
+        uint32_t gi = __builtin_amdgcn_readlane(ri.gi, j);
+        float rcp_w = 1.0f / float(w);
+        for (uint32_t t = lane; t < cj; t += 64) {
+          uint32_t tile = rect_tile(t, w, rcp_w, x0, y0, vd.tiles_x) - band;
+          if (tile < bw) {
+            uint32_t pos = atomicAdd(&cursor[tile], 1u);
+            uint64_t slot = ebase + pos;
+            if (slot < capacity) order[slot] = gi;
+            else over = true;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (over) atomicOr(overflow, 1u);
+}
+
+}  // namespace
+
+size_t radix_hist_elems(uint32_t n, uint32_t num_views) {
+  size_t chunks = (size_t(n) + kSortChunk - 1) / kSortChunk;
+  return size_t(num_views) * kBins * chunks;
+}
+
+size_t scan_piece_elems(size_t seg_len, uint32_t num_segs) {
+  return size_t(num_segs) * ((seg_len + kScanPiece - 1) / kScanPiece);
+}
+
+void launch_depth_sort(hipStream_t s, uint32_t* keys, uint32_t n, uint32_t num_views, const SortScratch& sc) {
+  if (n == 0 || num_views == 0) return;
+  const uint32_t chunks = (n + kSortChunk - 1) / kSortChunk;
+  dim3 grid(chunks, num_views);
+  uint32_t* kin = keys;
+  uint32_t* kout = sc.keys_alt;
+  const uint32_t* vin = nullptr;
+  uint32_t* vout = sc.vals_alt;
+  for (int pass = 0; pass < 32 / kRadixBits; ++pass) {
+    int shift = pass * kRadixBits;
+    hipLaunchKernelGGL(radix_hist_kernel, grid, dim3(kSortThreads), 0, s, kin, n, chunks, shift, sc.hist);
+    scan_segments(s, sc.hist, size_t(kBins) * chunks, num_views, sc.piece);
+    hipLaunchKernelGGL(radix_scatter_kernel, grid, dim3(kSortThreads), 0, s, kin, vin, n, chunks, shift, sc.hist,
+                       kout, vout);
+    // Ping-pong: after an even number of passes the keys are back in `keys` and
+    // the values in `vals`.
+    uint32_t* kt = kin; kin = kout; kout = kt;
+    vin = vout;
+    vout = (vout == sc.vals_alt) ? sc.vals : sc.vals_alt;
+  }
+}
+
+size_t bin_matrix_elems(const ViewDesc* host_views, uint32_t num_views, uint32_t n, uint32_t chunk) {
+  size_t chunks = (size_t(n) + chunk - 1) / chunk;
+  size_t total = 0;
+  for (uint32_t v = 0; v < num_views; ++v) total += chunks * host_views[v].tiles_x * host_views[v].tiles_y;
+  return total;
+}
+
+void launch_bin_count(hipStream_t s, const ViewDesc* views, uint32_t num_views, uint32_t n, uint32_t chunk,
+                      const uint32_t* keys, const uint32_t* vals, const uint2* rects, const BinScratch& bs,
+                      uint32_t* ranges, uint32_t max_tiles_per_view) {
+  if (num_views == 0) return;
+  const uint32_t chunks = n == 0 ? 0 : (n + chunk - 1) / chunk;
+  const uint32_t band = max_tiles_per_view < kBand ? max_tiles_per_view : kBand;
+  const size_t lds = size_t(band) * sizeof(uint32_t);
+  if (chunks > 0) {
+    hipLaunchKernelGGL(bin_count_kernel, dim3(chunks, num_views), dim3(64), lds, s, views, n, chunk, chunks, keys,
+                       vals, rects, bs.matrix_off, bs.matrix);
+    dim3 tgrid((max_tiles_per_view + 255) / 256, num_views);
+    hipLaunchKernelGGL(bin_colsum_kernel, tgrid, dim3(256), 0, s, views, chunks, bs.matrix_off, bs.matrix,
+                       bs.tile_tot);
+  }
+  hipLaunchKernelGGL(bin_tilescan_kernel, dim3(num_views), dim3(1024), 0, s, views, chunks, bs.tile_tot, ranges,
+                     bs.entries);
+  hipLaunchKernelGGL(bin_entry_base_kernel, dim3(1), dim3(64), 0, s, const_cast<ViewDesc*>(views), num_views,
+                     bs.entries);
+}
+
+void launch_bin_emit(hipStream_t s, const ViewDesc* views, uint32_t num_views, uint32_t n, uint32_t chunk,
+                     const uint32_t* keys, const uint32_t* vals, const uint2* rects, const BinScratch& bs,
+                     const uint32_t* ranges, uint32_t* order, uint64_t order_capacity, uint32_t* overflow,
+                     uint32_t max_tiles_per_view) {
+  const uint32_t chunks = n == 0 ? 0 : (n + chunk - 1) / chunk;
+  if (num_views == 0 || chunks == 0) return;
+  const uint32_t band = max_tiles_per_view < kBand ? max_tiles_per_view : kBand;
+  const size_t lds = size_t(band) * sizeof(uint32_t);
+  dim3 tgrid((max_tiles_per_view + 255) / 256, num_views);
+  hipLaunchKernelGGL(bin_colscan_kernel, tgrid, dim3(256), 0, s, views, chunks, bs.matrix_off, ranges, bs.matrix);
+  hipLaunchKernelGGL(bin_emit_kernel, dim3(chunks, num_views), dim3(64), lds, s, views, n, chunk, keys, vals, rects,
+                     bs.matrix_off, bs.matrix, order, order_capacity, overflow);
+}
+
+}  // namespace gsr
