@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Throughput of the differentiable Gaussian-splat rasterizer on one MI355X.
+
+Workload: 1M Gaussians (SH degree 3), 32 views of uneven resolution (uniform over
+320x240 / 640x480 / 1280x720 / 1920x1080) on a Fibonacci sphere of radius
+2.5-3.5 with horizontal FOV 45-75 degrees, L1 loss against seeded random
+targets.  One step = forward + backward of all 32 views through the batched
+path (one launch per stage, persistent tile queue, HIP graph).  Also timed: the
+plain path (the single-view C ABI called view after view), per-stage HIP-event
+times, and the FP32 CPU restatement of the reference on a bounded sample.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "home-measure-privacy_amd"))
+
+import numpy as np  # noqa: E402
+
+from gsr import lib, scenes  # noqa: E402
+
+METRIC = "views/s fwd+bwd, 1M Gaussians: 32-view batched vs 1-view plain; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+SH_BYTES = 192
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def sub(buf: lib.Buffer, offset: int, nbytes: int) -> lib.Buffer:
+    return lib.Buffer(buf.ctx, buf.ptr + offset, nbytes)
+
+
+def stage_bytes(sc, counters, n):
+    """Algorithmic bytes per stage (compulsory traffic only), for the whole batch."""
+    S = SH_BYTES if sc.sh_degree > 0 else 0
+    V = len(sc.cams)
+    P = np.array([c.pixels for c in sc.cams], dtype=np.float64)
+    T = np.array([c.tiles for c in sc.cams], dtype=np.float64)
+    E, F, G = (counters[:, k].astype(np.float64) for k in range(3))
+    return {
+        "project": n * (64 + S) + V * n * 48,
+        "bin_sort": float((n * 48 + E * 4 + T * 8).sum()),
+        "forward": float((F * (4 + 48) + T * 8 + P * 12).sum()),
+        "loss_grad": float((P * 36).sum()),
+        "backward": float((F * (4 + 48) + T * 8 + P * 12 + P * 12 + G * 36 * 2).sum()),
+        "chain": n * (64 + S) + V * n * 36 + n * (40 + S),
+    }
+
+
+def plain_path(ctx, sc, gsb, shb, tgt, n, max_entries, passes):
+    """The single-view pipeline of the C ABI, view after view (the reference's
+    one-view step, gpu.js:915-933), with buffers allocated once."""
+    maxp = max(c.pixels for c in sc.cams)
+    maxt = max(c.tiles for c in sc.cams)
+    splats = ctx.alloc(n * 48)
+    order = ctx.alloc(int(max_entries * 1.05) * 4 + 64)
+    ranges = ctx.alloc(maxt * 8)
+    image = ctx.alloc(maxp * 12)
+    dl = ctx.alloc(maxp * 12)
+    grads = ctx.alloc(n * 64)
+    shape = ctx.alloc(n * 40)
+    shg = ctx.alloc(n * 192)
+    offs = np.concatenate([[0], np.cumsum([c.pixels * 12 for c in sc.cams])])
+
+    def one(v):
+        cam = sc.cams[v]
+        ctx.project(cam, gsb, n, splats, shb, 3)
+        ctx.bin_sort(cam, splats, n, order, ranges)
+        ctx.forward(cam, splats, order, ranges, image)
+        ctx.loss_grad(cam, image, sub(tgt, int(offs[v]), cam.pixels * 12), 0.0, dl)
+        ctx.clear(grads)
+        ctx.backward(cam, splats, order, ranges, image, dl, grads)
+        ctx.shape(cam, gsb, n, grads, shape, shb, 3, shg)
+
+    for v in range(len(sc.cams)):  # warm
+        one(v)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        for v in range(len(sc.cams)):
+            one(v)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    for b in (splats, order, ranges, image, dl, grads, shape, shg):
+        b.free()
+    return passes * len(sc.cams) / dt
+
+
+def cpu_baseline(sc, budget_s: float):
+    """FP32 restatement of the reference's one-view step on the host cores
+    (project, tileLists, forward, L1 gradient, backward with atomic adds, SHAPE by
+    central differences), over the batch's views in order until budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    cores = O.threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    done, t0 = 0, time.perf_counter()
+    for cam, target in zip(sc.cams, sc.targets):
+        splats = O.project(cam, sc.gs.astype(np.float32), sc.sh.astype(np.float32), 3, np.float32)
+        lst, ranges = O.tile_lists(splats, cam.width, cam.height)
+        img, st = O.forward(splats, lst, ranges, cam.width, cam.height, np.float32)
+        dl = O.loss_grad(img, target, 0.0, np.float32)
+        grads = O.backward(splats, lst, ranges, st, dl, cam.width, cam.height, np.float32, threaded=True)
+        O.shape_fd(cam, sc.gs.astype(np.float32), grads, np.float32)
+        done += 1
+        log(f"  cpu view {done}: {cam.width}x{cam.height} E={lst.size} {time.perf_counter() - t0:.1f}s")
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    sizes = sorted({f"{c.width}x{c.height}" for c in sc.cams[:done]})
+    return {"value": done / dt, "unit": "views/s", "cores": cores, "kind": "port",
+            "sample": f"first {done} of the {len(sc.cams)} views ({', '.join(sizes)}), all {sc.n} Gaussians, "
+                      f"FP32 restatement of the reference step (SH3 colour, SHAPE by central differences), "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--gaussians", type=int, default=1_000_000)
+    ap.add_argument("--views", type=int, default=32)
+    ap.add_argument("--plain-passes", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design shards views and "
+              f"all-reduces gradients but is not built in this tree.", file=sys.stderr)
+        sys.exit(2)
+
+    n, V = args.gaussians, args.views
+    log(f"scene: {n} Gaussians, {V} views")
+    sc = scenes.config(3, n=n, views=V)
+    ctx = lib.Context(0)
+    gsb = ctx.upload(sc.gs)
+    shb = ctx.upload(sc.sh)
+    tgt = ctx.upload(np.concatenate([t.ravel() for t in sc.targets]).astype(np.float32))
+    grads, shape, shg = ctx.alloc(n * 64), ctx.alloc(n * 40), ctx.alloc(n * 192)
+    batch = lib.Batch(ctx, sc.cams, n, 3)
+    step = lambda: batch.step(gsb, shb, tgt, 0.0, grads, shape, shg, None, True)  # noqa: E731
+    step()
+    tile_batch, chunk = 1, 0
+    if not args.no_autotune:
+        tile_batch, chunk = batch.autotune(gsb, shb, tgt, 0.0, grads, shape, shg)
+        log(f"autotune: tile_batch={tile_batch} chunk={chunk}")
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    ms = dt / args.steps * 1e3
+    value = V / (dt / args.steps)
+    log(f"batched: {ms:.2f} ms/step, {value:.1f} views/s")
+
+    stage_ms, counters = batch.profile(gsb, shb, tgt, 0.0, grads, shape, shg)
+    log("stages (ms): " + ", ".join(f"{k}={v:.3f}" for k, v in stage_ms.items()))
+    bytes_ = stage_bytes(sc, counters, n)
+    stage_bw = {k: bytes_[k] / (stage_ms[k] * 1e-3) / 1e9 for k in bytes_ if k in stage_ms and stage_ms[k] > 0}
+    stage_bw["bin_sort"] = bytes_["bin_sort"] / (
+        (stage_ms["depth_sort"] + stage_ms["bin_count"] + stage_ms["bin_emit"]) * 1e-3) / 1e9
+    timed = {k: v for k, v in stage_ms.items()}
+    dominant = max(("forward", "backward", "project", "chain", "loss_grad"), key=lambda k: timed[k])
+    sort_ms = timed["depth_sort"] + timed["bin_count"] + timed["bin_emit"]
+    if sort_ms > timed[dominant]:
+        dominant = "bin_sort"
+    dom_ms = sort_ms if dominant == "bin_sort" else timed[dominant]
+    achieved = bytes_[dominant] / (dom_ms * 1e-3) / 1e9
+    bwd_achieved = bytes_["backward"] / (timed["backward"] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{dominant}.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("bytes_per_launch")
+
+    plain = None
+    if not args.no_plain:
+        plain = plain_path(ctx, sc, gsb, shb, tgt, n, int(counters[:, 0].max()), args.plain_passes)
+        log(f"plain: {plain:.1f} views/s  (batched / plain = {value / plain:.2f}x)")
+    batch.close()
+    for b in (gsb, shb, tgt, grads, shape, shg):
+        b.free()
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(sc, args.cpu_seconds)
+        log(f"cpu baseline: {cpu['value']:.4f} views/s on {cpu['cores']} threads")
+
+    E, F, G = counters.sum(0)
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "views/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {
+            "workload": f"{n} Gaussians SH3, {V} views uneven (320x240..1920x1080), Fibonacci sphere r 2.5-3.5, "
+                        f"FOV 45-75, L1 vs seeded target; one step = fwd+bwd of all views",
+            "gaussians": n, "views": V, "sh_degree": 3, "pixels_per_step": int(sum(c.pixels for c in sc.cams)),
+            "entries_per_step": int(E), "saturated_entries_per_step": int(F), "touched_gaussians_per_step": int(G),
+            "plain_views_per_s": None if plain is None else round(plain, 3),
+            "batched_over_plain": None if plain is None else round(value / plain, 3),
+            "gaussians_projected_per_s": round(n * V / (timed["project"] * 1e-3), 1),
+            "stage_ms": {k: round(v, 4) for k, v in timed.items()},
+            "stage_GBps": {k: round(v, 1) for k, v in stage_bw.items()},
+            "backward_hbm_frac": round(bwd_achieved / HBM_PEAK_GBS, 4),
+            "graph": True, "tile_batch": tile_batch, "chunk": chunk,
+        },
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

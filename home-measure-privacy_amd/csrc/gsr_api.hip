@@ -303,6 +303,7 @@ static int raster_common(gsr_ctx* ctx, const gsr_camera* cam, RasterArgs& a, Vie
   a.grads_view_stride = 0;
   a.queue = nullptr;
   a.tile_batch = 1;
+  a.order_capacity = ~uint64_t(0);  // the caller's order holds every entry its ranges name
   return GSR_OK;
 }
 

@@ -169,25 +169,42 @@ int upload_cams(gsr_batch* b) {
   return GSR_OK;
 }
 
+// Stage boundaries and workload counters of a profiled step.
+struct Prof {
+  hipEvent_t ev[GSR_STAGES + 1];
+  uint32_t* tile_last;
+  uint32_t* touched;
+};
+
 // Issues one full step on the stream (capturable: kernels only).
 void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* targets, float lambda, float* grads,
-                float* shape, float* sh_grad, float* images) {
+                float* shape, float* sh_grad, float* images, Prof* prof = nullptr) {
   hipStream_t s = ctx_stream(b->ctx);
   float* image = images ? images : b->image;
+  int stage = 0;
+  auto mark = [&]() {
+    if (prof) hipEventRecord(prof->ev[stage++], s);
+  };
   launch_zero(s, b->flags, 16);
+  mark();
   launch_project(s, b->d_cams, b->d_views, b->V, gs, sh, b->deg, b->n, b->splats, b->keys, b->rects, nullptr);
+  mark();
   SortScratch sc{b->keys_alt, b->vals, b->vals_alt, b->hist, b->piece};
   launch_depth_sort(s, b->keys, b->n, b->V, sc);
+  mark();
   BinScratch bs{b->matrix, b->matrix_off, b->tile_tot, b->entries};
   launch_bin_count(s, b->d_views, b->V, b->n, b->chunk, b->keys, b->vals, b->rects, bs, b->ranges, b->max_tiles);
+  mark();
   launch_bin_emit(s, b->d_views, b->V, b->n, b->chunk, b->keys, b->vals, b->rects, bs, b->ranges, b->order,
                   b->order_cap, b->flags, b->max_tiles);
+  mark();
   RasterArgs a{};
   a.views = b->d_views;
   a.work = b->d_work;
   a.num_work = b->total_tiles;
   a.splats = b->splats;
   a.order = b->order;
+  a.order_capacity = b->order_cap;
   a.ranges = b->ranges;
   a.image = image;
   a.state = nullptr;
@@ -196,11 +213,18 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   a.grads_view_stride = uint64_t(b->n) * 16;
   a.tile_batch = b->tile_batch;
   a.queue = b->flags + 1;
+  a.tile_last = prof ? prof->tile_last : nullptr;
   launch_forward(s, a, 1, b->grid_fwd);
+  a.tile_last = nullptr;
+  mark();
   launch_loss_grad(s, b->d_views, b->views.data(), b->V, image, targets, lambda, b->coef, b->dl);
+  mark();
   a.queue = b->flags + 2;
   launch_backward(s, a, 1, b->grid_bwd);
-  launch_shape(s, b->d_cams, b->V, gs, sh, b->deg, b->n, b->vgrads, uint64_t(b->n) * 16, shape, sh_grad, grads, 1);
+  mark();
+  launch_shape(s, b->d_cams, b->V, gs, sh, b->deg, b->n, b->vgrads, uint64_t(b->n) * 16, shape, sh_grad, grads, 1,
+               prof ? prof->touched : nullptr);
+  mark();
 }
 
 int run_step(gsr_batch* b, const float* gs, const float* sh, const float* targets, float lambda, float* grads,
@@ -350,6 +374,47 @@ int gsr_batch_configure(gsr_batch* b, uint32_t tile_batch, uint32_t chunk) {
     if (chunk < 64 || chunk > 65536) return ctx_fail(b->ctx, GSR_E_INVALID, "chunk not in [64, 65536]");
     return set_chunk(b, chunk);
   }
+  return GSR_OK;
+}
+
+int gsr_batch_profile(gsr_batch* b, const float* gs, const float* sh, const float* targets, float lambda,
+                      float* grads, float* shape, float* sh_grad, float* stage_ms, uint64_t* counters) {
+  if (!b || !stage_ms) return GSR_E_INVALID;
+  // A regular step first: sizes the entry storage for these inputs.
+  int rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 0);
+  if (rc) return rc;
+  hipStream_t s = ctx_stream(b->ctx);
+  Prof prof{};
+  std::vector<uint32_t> tl(b->total_tiles), touched(b->V);
+  if (hipMalloc(reinterpret_cast<void**>(&prof.tile_last), std::max<size_t>(4, tl.size() * 4)) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&prof.touched), b->V * 4) != hipSuccess) {
+    dfree(prof.tile_last);
+    return ctx_fail(b->ctx, GSR_E_NOMEM, "profile buffers");
+  }
+  launch_zero(s, prof.touched, b->V * 4);
+  for (auto& e : prof.ev) hipEventCreate(&e);
+  issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, &prof);
+  hipError_t e = hipStreamSynchronize(s);
+  if (e == hipSuccess) {
+    for (int k = 0; k < GSR_STAGES; ++k) hipEventElapsedTime(stage_ms + k, prof.ev[k], prof.ev[k + 1]);
+    if (counters) {
+      hipMemcpy(tl.data(), prof.tile_last, tl.size() * 4, hipMemcpyDeviceToHost);
+      hipMemcpy(touched.data(), prof.touched, b->V * 4, hipMemcpyDeviceToHost);
+      std::vector<uint64_t> ent(b->V);
+      hipMemcpy(ent.data(), b->entries, b->V * 8, hipMemcpyDeviceToHost);
+      for (uint32_t v = 0; v < b->V; ++v) {
+        uint64_t f = 0;
+        for (uint32_t t = 0; t < b->views[v].num_tiles; ++t) f += tl[b->views[v].tile_base + t];
+        counters[3 * v] = ent[v];
+        counters[3 * v + 1] = f;
+        counters[3 * v + 2] = touched[v];
+      }
+    }
+  }
+  for (auto& ev : prof.ev) hipEventDestroy(ev);
+  dfree(prof.tile_last);
+  dfree(prof.touched);
+  if (e != hipSuccess) return ctx_fail(b->ctx, GSR_E_HIP, "profiled step failed");
   return GSR_OK;
 }
 

@@ -18,7 +18,7 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
                                                     int deg, uint32_t n, float* __restrict__ grads,
                                                     uint64_t stride, float* __restrict__ shape,
                                                     float* __restrict__ sh_grad, float* __restrict__ grads_out,
-                                                    int consume) {
+                                                    int consume, uint32_t* __restrict__ touched) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4* g4 = reinterpret_cast<const float4*>(gs) + size_t(i) * 4;
@@ -51,10 +51,12 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
     gop += gr[3];
     bool screen = gr[4] != 0.0f || gr[5] != 0.0f || gr[6] != 0.0f || gr[7] != 0.0f || gr[8] != 0.0f;
     bool colour = deg > 0 && (gr[0] != 0.0f || gr[1] != 0.0f || gr[2] != 0.0f);
-    if (consume && (screen || gr[0] != 0.0f || gr[1] != 0.0f || gr[2] != 0.0f || gr[3] != 0.0f)) {
+    const bool any = screen || gr[0] != 0.0f || gr[1] != 0.0f || gr[2] != 0.0f || gr[3] != 0.0f;
+    if (consume && any) {
       const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
       r[0] = z; r[1] = z; r[2] = z;
     }
+    if (touched && any) atomicAdd(touched + v, 1u);  // profiling only
     if (!screen && !colour) continue;
     const CamF cam = load_cam(cams[v]);
     float p[3];
@@ -252,10 +254,10 @@ __global__ __launch_bounds__(256) void stats_kernel(uint32_t n, const float* __r
 
 void launch_shape(hipStream_t s, const gsr_camera* cams, uint32_t num_views, const float* gs, const float* sh,
                   int sh_degree, uint32_t n, float* grads, uint64_t grads_view_stride, float* shape,
-                  float* sh_grad, float* grads_out, int consume) {
+                  float* sh_grad, float* grads_out, int consume, uint32_t* touched) {
   if (n == 0) return;
   hipLaunchKernelGGL(shape_kernel, dim3((n + 255) / 256), dim3(256), 0, s, cams, num_views, gs, sh, sh_degree, n,
-                     grads, grads_view_stride, shape, sh_grad, grads_out, consume);
+                     grads, grads_view_stride, shape, sh_grad, grads_out, consume, touched);
 }
 
 void launch_adam(hipStream_t s, const gsr_step& step, float* gs, const float* grads, const float* shape,

@@ -93,6 +93,7 @@ struct RasterArgs {
   uint32_t num_work;
   const float* splats;
   const uint32_t* order;
+  uint64_t order_capacity;  // slots in order; tiles whose range ends beyond it are skipped
   const uint32_t* ranges;
   float* image;          // concatenated views
   float* state;          // optional
@@ -101,6 +102,7 @@ struct RasterArgs {
   uint64_t grads_view_stride;  // elements between views' gradient records (0: shared)
   uint32_t* queue;       // persistent mode: work counter (zeroed before launch)
   uint32_t tile_batch;   // tiles claimed per queue pop
+  uint32_t* tile_last;   // forward, optional: per global tile, furthest last hit - start
 };
 void launch_forward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid);
 void launch_backward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid);
@@ -116,9 +118,10 @@ void launch_zero(hipStream_t s, void* p, size_t bytes);
 // receives the summed colour (0-2) and opacity-logit (3) gradients.  With
 // consume != 0 every per-view record that held a gradient is zeroed after use,
 // which leaves the per-view accumulators clear for the next backward.
+// touched (optional, [num_views]) counts the Gaussians with any gradient per view.
 void launch_shape(hipStream_t s, const gsr_camera* cams, uint32_t num_views, const float* gs, const float* sh,
                   int sh_degree, uint32_t n, float* grads, uint64_t grads_view_stride, float* shape,
-                  float* sh_grad, float* grads_out, int consume);
+                  float* sh_grad, float* grads_out, int consume, uint32_t* touched = nullptr);
 void launch_adam(hipStream_t s, const gsr_step& step, float* gs, const float* grads, const float* shape,
                  float* moments);
 void launch_stats(hipStream_t s, const gsr_step& step, const float* grads, const float* splats, float* stats);

@@ -46,6 +46,9 @@ __device__ inline Tile load_tile(const RasterArgs& a, uint32_t w) {
   t.gtile = vd.tile_base + ty * vd.tiles_x + tx;
   t.start = a.ranges[size_t(t.gtile) * 2];
   t.end = a.ranges[size_t(t.gtile) * 2 + 1];
+  // A batched step whose entries overflowed the order storage is rerun; its
+  // ranges then point past the storage, so such tiles read nothing.
+  if (t.end < t.start || vd.entry_base + t.end > a.order_capacity) t.start = t.end = 0;
   t.order = a.order + vd.entry_base;
   t.splats = a.splats + vd.splat_base * 12;
   t.pixel_base = vd.pixel_base;
@@ -138,6 +141,14 @@ __device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) 
         last[i] = k0 + j + 1;
       }
     }
+  }
+  if (a.tile_last) {
+    // Profiling: the tile's saturated prefix, furthest last hit minus start.
+    uint32_t m = t.start;
+#pragma unroll
+    for (int i = 0; i < kPix; ++i) m = max(m, last[i]);
+    for (int off = 32; off > 0; off >>= 1) m = max(m, uint32_t(__shfl_xor(int(m), off)));
+    if (lane == 0) a.tile_last[t.gtile] = m - t.start;
   }
 #pragma unroll
   for (int i = 0; i < kPix; ++i) {

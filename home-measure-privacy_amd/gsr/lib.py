@@ -23,8 +23,9 @@ EXPORTS = (
     "gsr_create", "gsr_destroy", "gsr_last_error", "gsr_alloc", "gsr_free", "gsr_upload", "gsr_download",
     "gsr_clear", "gsr_synchronize", "gsr_project", "gsr_bin_sort", "gsr_forward", "gsr_loss_grad", "gsr_backward",
     "gsr_shape", "gsr_adam", "gsr_stats", "gsr_batch_create", "gsr_batch_destroy", "gsr_batch_set_cameras",
-    "gsr_batch_step", "gsr_batch_entries", "gsr_batch_configure", "gsr_batch_autotune",
+    "gsr_batch_step", "gsr_batch_entries", "gsr_batch_configure", "gsr_batch_autotune", "gsr_batch_profile",
 )
+STAGES = ("project", "depth_sort", "bin_count", "bin_emit", "forward", "loss_grad", "backward", "chain")
 
 
 class GsrError(RuntimeError):
@@ -84,6 +85,7 @@ def load():
         "gsr_batch_entries": (i32, [P, ctypes.POINTER(u64)]),
         "gsr_batch_configure": (i32, [P, u32, u32]),
         "gsr_batch_autotune": (i32, [P, P, P, P, f32, P, P, P, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+        "gsr_batch_profile": (i32, [P, P, P, P, f32, P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -254,6 +256,15 @@ class Batch:
 
     def configure(self, tile_batch: int = 0, chunk: int = 0):
         self.ctx._check(self.lib.gsr_batch_configure(self.h, tile_batch, chunk))
+
+    def profile(self, gs, sh, targets, lam, grads, shape, sh_grad=None):
+        """One eager step timed per stage with HIP events: (ms per stage dict,
+        counters [V, 3] of entries E, saturated-prefix entries F, touched G)."""
+        ms = (ctypes.c_float * len(STAGES))()
+        cnt = (ctypes.c_uint64 * (3 * self.V))()
+        self.ctx._check(self.lib.gsr_batch_profile(self.h, gs.p, sh.p if sh else None, targets.p, lam, grads.p,
+                                                   shape.p, sh_grad.p if sh_grad else None, ms, cnt))
+        return dict(zip(STAGES, [float(x) for x in ms])), np.array(cnt[:], dtype=np.uint64).reshape(self.V, 3)
 
     def autotune(self, gs, sh, targets, lam, grads, shape, sh_grad=None):
         tb, ch = ctypes.c_uint32(0), ctypes.c_uint32(0)

@@ -165,6 +165,15 @@ int gsr_batch_entries(gsr_batch* batch, uint64_t* entries);
  * binning chunk (0 = keep).  gsr_batch_autotune times candidates on the given
  * inputs and keeps the fastest; it writes the choice to *tile_batch / *chunk. */
 int gsr_batch_configure(gsr_batch* batch, uint32_t tile_batch, uint32_t chunk);
+/* Runs one step without the graph, timing each stage with HIP events on the
+ * context's stream.  stage_ms[GSR_STAGES] receives milliseconds for: project,
+ * depth sort, bin count, bin emit, forward, loss gradient, backward, chain.
+ * counters[3 * num_views] receives per view: entries E, saturated-prefix entries
+ * F (sum over tiles of the furthest pixel's last hit minus the tile start) and
+ * touched Gaussians G (any gradient in that view). */
+#define GSR_STAGES 8
+int gsr_batch_profile(gsr_batch* batch, const float* gs, const float* sh, const float* targets, float lambda,
+                      float* grads, float* shape, float* sh_grad, float* stage_ms, uint64_t* counters);
 int gsr_batch_autotune(gsr_batch* batch, const float* gs, const float* sh, const float* targets, float lambda,
                        float* grads, float* shape, float* sh_grad, uint32_t* tile_batch, uint32_t* chunk);
 

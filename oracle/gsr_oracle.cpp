@@ -519,6 +519,56 @@ void backward(const R* splats, const uint32_t* list, const uint32_t* ranges, con
   }
 }
 
+// The same per-pixel backward run on every host thread, gradients added with
+// atomic float adds (a CAS loop, as addF, gpu.js:148-157).  Summation order then
+// depends on scheduling, as on the GPU; used to time the CPU baseline.
+template <class R>
+void backward_mt(const R* splats, const uint32_t* list, const uint32_t* ranges, const R* state, const R* dLdpix,
+                 int W, int H, R* grads) {
+  const int tilesX = (W + 15) / 16;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long pix = 0; pix < long(W) * H; ++pix) {
+    int x = int(pix % W), y = int(pix / W);
+    R px = R(x) + R(0.5), py = R(y) + R(0.5);
+    R dL[3] = {dLdpix[pix * 3], dLdpix[pix * 3 + 1], dLdpix[pix * 3 + 2]};
+    int tile = (y / 16) * tilesX + x / 16;
+    uint32_t start = ranges[tile * 2];
+    R T = state[pix * 2];
+    R behind[3] = {0, 0, 0};
+    uint32_t k = uint32_t(state[pix * 2 + 1]);
+    while (k > start) {
+      k -= 1;
+      uint32_t gi = list[k];
+      const R* o = splats + size_t(gi) * 12;
+      R alpha, dx, dy;
+      if (!alpha_at(o, px, py, alpha, dx, dy)) continue;
+      R Tk = T / (R(1) - alpha);
+      R g[9] = {dL[0] * alpha * Tk, dL[1] * alpha * Tk, dL[2] * alpha * Tk, 0, 0, 0, 0, 0, 0};
+      R dAlpha = 0;
+      for (int c = 0; c < 3; ++c) dAlpha += dL[c] * (Tk * o[8 + c] - behind[c] / (R(1) - alpha));
+      for (int c = 0; c < 3; ++c) behind[c] += o[8 + c] * alpha * Tk;
+      T = Tk;
+      if (alpha < R(0.99)) {
+        R op = o[5];
+        g[3] = dAlpha * (alpha / op) * op * (R(1) - op);
+        R dPower = dAlpha * alpha;
+        R A = o[2], B = o[3], C = o[4];
+        g[4] = dPower * (A * dx + B * dy);
+        g[5] = dPower * (C * dy + B * dx);
+        g[6] = dPower * R(-0.5) * dx * dx;
+        g[7] = dPower * -dx * dy;
+        g[8] = dPower * R(-0.5) * dy * dy;
+      }
+      R* out = grads + size_t(gi) * 16;
+      for (int q = 0; q < 9; ++q) {
+        if (g[q] == R(0)) continue;
+#pragma omp atomic
+        out[q] += g[q];
+      }
+    }
+  }
+}
+
 // Analytic chain from screen-space gradients (grads[4..8] = u, v, conic A, B, C)
 // to mean, log-scale and rotation: the derivative of projectShape (gpu.js:30-60)
 // that SHAPE (gpu.js:297-328) estimates by central differences.  With deg > 0 the
@@ -756,6 +806,10 @@ int oracle_num_threads(int n) {
   void oracle_backward_##SUF(const R* splats, const uint32_t* list, const uint32_t* ranges, const R* state,          \
                              const R* dL, int W, int H, R* grads) {                                                  \
     backward<R>(splats, list, ranges, state, dL, W, H, grads);                                                       \
+  }                                                                                                                   \
+  void oracle_backward_mt_##SUF(const R* splats, const uint32_t* list, const uint32_t* ranges, const R* state,       \
+                                const R* dL, int W, int H, R* grads) {                                               \
+    backward_mt<R>(splats, list, ranges, state, dL, W, H, grads);                                                    \
   }                                                                                                                   \
   void oracle_shape_grad_##SUF(const CamU* cam, const R* gs, const R* sh, int deg, uint32_t n,                        \
                                const R* grads, R* shape, R* sh_grad) {                                               \

@@ -126,12 +126,14 @@ def loss_value(image, target, lam: float, dtype=np.float64) -> float:
     return float(getattr(lib(), f"oracle_loss_value_{_suf(dtype)}")(_p(image), _p(target), W, H, lam))
 
 
-def backward(splats, lst, ranges, state, dL, width: int, height: int, dtype=np.float64):
+def backward(splats, lst, ranges, state, dL, width: int, height: int, dtype=np.float64, threaded: bool = False):
+    """BACKWARD (gpu.js:159-202).  threaded=True runs every host thread with
+    atomic float adds (nondeterministic order; CPU baseline timing)."""
     s = _c(splats, dtype)
     n = s.shape[0]
     lst = _c(lst if lst.size else np.zeros(1, np.uint32), np.uint32)
     grads = np.zeros((n, 16), dtype=dtype)
-    getattr(lib(), f"oracle_backward_{_suf(dtype)}")(_p(s), _p(lst), _p(_c(ranges, np.uint32)), _p(_c(state, dtype)),
+    getattr(lib(), f"oracle_backward_{'mt_' if threaded else ''}{_suf(dtype)}")(_p(s), _p(lst), _p(_c(ranges, np.uint32)), _p(_c(state, dtype)),
                                                      _p(_c(dL, dtype)), ctypes.c_int(width), ctypes.c_int(height),
                                                      _p(grads))
     return grads

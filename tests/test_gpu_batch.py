@@ -1,0 +1,135 @@
+"""Batched multi-view step against the single-view path of the same library and
+against the FP64 oracle summed over views."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rel_l2
+from gsr import lib, scenes
+from gsr.renderer import Renderer
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def small_batch_scene(n=20000, views=5, seed=0):
+    sc = scenes.config(3, n=n, views=views, seed=seed)
+    # shrink the views so the FP64 oracle stays fast; keep them uneven
+    sizes = [(96, 64), (160, 120), (64, 48), (200, 100), (33, 17), (128, 128), (48, 80)]
+    cams = []
+    for v, c in enumerate(sc.cams):
+        w, h = sizes[v % len(sizes)]
+        s = w / c.width
+        cams.append(scenes.Camera(c.view, c.fx * s, c.fx * s, w / 2.0, h / 2.0, w, h))
+    sc.cams = cams
+    rng = scenes.Rng(99 + seed)
+    sc.targets = [rng.uniform(c.pixels * 3).astype(np.float32).reshape(c.height, c.width, 3) for c in cams]
+    return sc
+
+
+def run_batch(ctx, sc, graph=True, tile_batch=0, chunk=0, steps=1, images=False):
+    n = sc.n
+    gsb, shb = ctx.upload(sc.gs), ctx.upload(sc.sh)
+    tgt = ctx.upload(np.concatenate([t.ravel() for t in sc.targets]).astype(np.float32))
+    grads, shape, shg = ctx.alloc(n * 64), ctx.alloc(n * 40), ctx.alloc(n * 192)
+    total_px = sum(c.pixels for c in sc.cams)
+    img = ctx.alloc(total_px * 12) if images else None
+    b = lib.Batch(ctx, sc.cams, n, 3)
+    b.configure(tile_batch, chunk)
+    outs = []
+    for _ in range(steps):
+        b.step(gsb, shb, tgt, 0.0, grads, shape, shg, img, graph)
+        outs.append((ctx.download(grads, np.float32, (n, 16)), ctx.download(shape, np.float32, (n, 10)),
+                     ctx.download(shg, np.float32, (n, 16, 3)),
+                     ctx.download(img, np.float32, (total_px * 3,)) if images else None))
+    entries = b.entries()
+    b.close()
+    for x in (gsb, shb, tgt, grads, shape, shg, img):
+        if x is not None:
+            x.free()
+    return outs, entries
+
+
+def test_batch_matches_single_view_path(ctx):
+    sc = small_batch_scene()
+    log(f"batch vs single: n={sc.n} views={[(c.width, c.height) for c in sc.cams]}")
+    outs, entries = run_batch(ctx, sc, graph=False, images=True)
+    g_b, s_b, sh_b, img_b = outs[0]
+    r = Renderer()
+    g_sum = np.zeros((sc.n, 16))
+    s_sum = np.zeros((sc.n, 10))
+    sh_sum = np.zeros((sc.n, 16, 3))
+    imgs, ents = [], []
+    for cam, tgt in zip(sc.cams, sc.targets):
+        out = r.gradients(sc.gs, cam, tgt, 0.0, sh=sc.sh, deg=3)
+        g_sum += out["grads"]
+        s_sum += out["shape"]
+        sh_sum += out["sh_grad"]
+        imgs.append(out["image"].ravel())
+        ents.append(out["entries"])
+    r.close()
+    np.testing.assert_array_equal(entries, np.array(ents, dtype=np.uint64))
+    # same projection, ordering and blend arithmetic: identical images
+    np.testing.assert_array_equal(img_b, np.concatenate(imgs))
+    # gradients differ only by float atomic summation order
+    assert rel_l2(g_b[:, :4], g_sum[:, :4]) < 1e-5
+    assert rel_l2(s_b, s_sum) < 1e-5
+    assert rel_l2(sh_b, sh_sum) < 1e-5
+
+
+def test_batch_graph_replay_and_knobs(ctx):
+    """Graph replays (several steps) agree with the eager step; tile batch and
+    chunk size do not change results beyond summation order."""
+    sc = small_batch_scene(n=15000, views=4, seed=1)
+    eager, e0 = run_batch(ctx, sc, graph=False)
+    replay, e1 = run_batch(ctx, sc, graph=True, steps=3)
+    tuned, e2 = run_batch(ctx, sc, graph=True, tile_batch=4, chunk=512, steps=2)
+    np.testing.assert_array_equal(e0, e1)
+    np.testing.assert_array_equal(e0, e2)
+    for outs in (replay, tuned):
+        for g, s, sh, _ in outs:
+            assert rel_l2(g[:, :4], eager[0][0][:, :4]) < 1e-5
+            assert rel_l2(s, eager[0][1]) < 1e-5
+            assert rel_l2(sh, eager[0][2]) < 1e-5
+
+
+def test_batch_matches_fp64_oracle(ctx):
+    sc = small_batch_scene(n=6000, views=3, seed=2)
+    outs, _ = run_batch(ctx, sc, graph=True)
+    g_b, s_b, sh_b, _ = outs[0]
+    g_sum = np.zeros((sc.n, 16))
+    s_sum = np.zeros((sc.n, 10))
+    sh_sum = np.zeros((sc.n, 16, 3))
+    gs64, sh64 = sc.gs.astype(np.float64), sc.sh.astype(np.float64)
+    for cam, tgt in zip(sc.cams, sc.targets):
+        img = O.render(cam, gs64, sh64, 3)[3]
+        dl = O.loss_grad(img, tgt.astype(np.float64), 0.0)
+        _, grads, shape, shg = O.gradients(cam, gs64, dl, sh64, 3)
+        g_sum += grads
+        s_sum += shape
+        sh_sum += shg
+    # L1's sign(x - y) can flip on pixels where the float32 and float64 renders
+    # straddle the target; the bar is the end-to-end 1e-4 of the differentiation check.
+    for name, a, b in (("opacity", g_b[:, 3], g_sum[:, 3]), ("means", s_b[:, :3], s_sum[:, :3]),
+                       ("log_scales", s_b[:, 3:6], s_sum[:, 3:6]), ("rotations", s_b[:, 6:], s_sum[:, 6:]),
+                       ("sh", sh_b, sh_sum)):
+        e = rel_l2(a, b)
+        log(f"  batch vs fp64 {name}: {e:.2e}")
+        assert e < 1e-4, name
+
+
+def test_batch_entry_storage_grows(ctx):
+    """The first guess of entry storage is exceeded by large footprints; the step
+    detects the overflow, grows storage and reruns."""
+    sc = small_batch_scene(n=3000, views=2, seed=3)
+    sc.gs[:, 4:7] += 2.5  # much larger Gaussians: many entries per Gaussian
+    outs, entries = run_batch(ctx, sc, graph=True, steps=2)
+    assert int(entries.sum()) > 16 * sc.n * 2
+    r = Renderer()
+    for cam, e in zip(sc.cams, entries):
+        out = r.render(sc.gs, cam, sc.sh, 3)
+        assert out.shape == (cam.height, cam.width, 3)
+    r.close()
