@@ -56,14 +56,23 @@ def stage_bytes(sc, counters, n):
     }
 
 
-def plain_path(ctx, sc, gsb, shb, tgt, n, max_entries, passes):
+def plain_path(ctx, sc, gsb, shb, tgt, n, passes):
     """The single-view pipeline of the C ABI, view after view (the reference's
     one-view step, gpu.js:915-933), with buffers allocated once."""
     maxp = max(c.pixels for c in sc.cams)
     maxt = max(c.tiles for c in sc.cams)
     splats = ctx.alloc(n * 48)
-    order = ctx.alloc(int(max_entries * 1.05) * 4 + 64)
     ranges = ctx.alloc(maxt * 8)
+    max_entries = 0
+    for cam in sc.cams:  # full tile-list sizes, for the order buffer
+        ctx.project(cam, gsb, n, splats, shb, 3)
+        try:
+            max_entries = max(max_entries, ctx.bin_sort(cam, splats, n, None, ranges))
+        except lib.GsrError as e:
+            if e.code != lib.GSR_E_CAPACITY:
+                raise
+            max_entries = max(max_entries, e.entries)
+    order = ctx.alloc(max_entries * 4 + 64)
     image = ctx.alloc(maxp * 12)
     dl = ctx.alloc(maxp * 12)
     grads = ctx.alloc(n * 64)
@@ -131,6 +140,8 @@ def main():
     ap.add_argument("--gaussians", type=int, default=1_000_000)
     ap.add_argument("--views", type=int, default=32)
     ap.add_argument("--plain-passes", type=int, default=1)
+    ap.add_argument("--passes", type=int, default=0, help="depth-slice passes (0 = library default)")
+    ap.add_argument("--growth", type=int, default=8, help="growth of the depth-slice ends")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -150,6 +161,8 @@ def main():
     tgt = ctx.upload(np.concatenate([t.ravel() for t in sc.targets]).astype(np.float32))
     grads, shape, shg = ctx.alloc(n * 64), ctx.alloc(n * 40), ctx.alloc(n * 192)
     batch = lib.Batch(ctx, sc.cams, n, 3)
+    if args.passes:
+        batch.set_passes(args.passes, args.growth)
     step = lambda: batch.step(gsb, shb, tgt, 0.0, grads, shape, shg, None, True)  # noqa: E731
     step()
     tile_batch, chunk = 1, 0
@@ -189,7 +202,7 @@ def main():
 
     plain = None
     if not args.no_plain:
-        plain = plain_path(ctx, sc, gsb, shb, tgt, n, int(counters[:, 0].max()), args.plain_passes)
+        plain = plain_path(ctx, sc, gsb, shb, tgt, n, args.plain_passes)
         log(f"plain: {plain:.1f} views/s  (batched / plain = {value / plain:.2f}x)")
     batch.close()
     for b in (gsb, shb, tgt, grads, shape, shg):
@@ -201,7 +214,7 @@ def main():
         cpu = cpu_baseline(sc, args.cpu_seconds)
         log(f"cpu baseline: {cpu['value']:.4f} views/s on {cpu['cores']} threads")
 
-    E, F, G = counters.sum(0)
+    E, F, G, Wk = counters.sum(0)
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "views/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
@@ -211,6 +224,7 @@ def main():
                         f"FOV 45-75, L1 vs seeded target; one step = fwd+bwd of all views",
             "gaussians": n, "views": V, "sh_degree": 3, "pixels_per_step": int(sum(c.pixels for c in sc.cams)),
             "entries_per_step": int(E), "saturated_entries_per_step": int(F), "touched_gaussians_per_step": int(G),
+            "walked_entries_per_step": int(Wk),
             "plain_views_per_s": None if plain is None else round(plain, 3),
             "batched_over_plain": None if plain is None else round(value / plain, 3),
             "gaussians_projected_per_s": round(n * V / (timed["project"] * 1e-3), 1),
@@ -218,6 +232,7 @@ def main():
             "stage_GBps": {k: round(v, 1) for k, v in stage_bw.items()},
             "backward_hbm_frac": round(bwd_achieved / HBM_PEAK_GBS, 4),
             "graph": True, "tile_batch": tile_batch, "chunk": chunk,
+            "depth_passes": args.passes or "default",
         },
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},

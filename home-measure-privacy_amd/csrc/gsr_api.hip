@@ -92,7 +92,6 @@ ViewDesc view_of(const gsr_camera& c, uint32_t tile_base, uint64_t pixel_base, u
   v.num_tiles = v.tiles_x * v.tiles_y;
   v.pixel_base = pixel_base;
   v.splat_base = splat_base;
-  v.entry_base = 0;
   return v;
 }
 
@@ -256,27 +255,29 @@ int gsr_bin_sort(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, uint3
   if (rc) return rc;
   const size_t nn = std::max<uint32_t>(n, 1);
   uint32_t *keys, *keys_alt, *vals, *vals_alt, *hist, *piece, *matrix, *tile_tot, *overflow;
-  uint2* rects;
-  uint64_t *matrix_off, *entries;
+  uint2 *rects, *rect_sorted;
+  uint64_t *ebase, *entries, *run;
   const uint32_t chunk = pick_chunk(n);
   const size_t hist_n = std::max<size_t>(1, radix_hist_elems(n, 1));
   const size_t piece_n = std::max<size_t>(1, scan_piece_elems(hist_n, 1));
   const size_t mat_n = std::max<size_t>(1, bin_matrix_elems(&host, 1, n, chunk));
   if ((rc = pooled_t(ctx, "keys", nn, &keys)) || (rc = pooled_t(ctx, "keys_alt", nn, &keys_alt)) ||
       (rc = pooled_t(ctx, "vals", nn, &vals)) || (rc = pooled_t(ctx, "vals_alt", nn, &vals_alt)) ||
-      (rc = pooled_t(ctx, "rects", nn, &rects)) || (rc = pooled_t(ctx, "hist", hist_n, &hist)) ||
+      (rc = pooled_t(ctx, "rects", nn, &rects)) || (rc = pooled_t(ctx, "rect_sorted", nn, &rect_sorted)) ||
+      (rc = pooled_t(ctx, "hist", hist_n, &hist)) ||
       (rc = pooled_t(ctx, "piece", piece_n, &piece)) || (rc = pooled_t(ctx, "matrix", mat_n, &matrix)) ||
-      (rc = pooled_t(ctx, "matrix_off", 1, &matrix_off)) ||
       (rc = pooled_t(ctx, "tile_tot", host.num_tiles, &tile_tot)) || (rc = pooled_t(ctx, "entries", 1, &entries)) ||
+      (rc = pooled_t(ctx, "ebase", 1, &ebase)) || (rc = pooled_t(ctx, "run", 1, &run)) ||
       (rc = pooled_t(ctx, "overflow", 1, &overflow)))
     return rc;
-  launch_zero(ctx->stream, matrix_off, sizeof(uint64_t));
   launch_zero(ctx->stream, overflow, sizeof(uint32_t));
   launch_prep(ctx->stream, dview, 1, splats, n, keys, rects);
   SortScratch sc{keys_alt, vals, vals_alt, hist, piece};
   launch_depth_sort(ctx->stream, keys, n, 1, sc);
-  BinScratch bs{matrix, matrix_off, tile_tot, entries};
-  launch_bin_count(ctx->stream, dview, 1, n, chunk, keys, vals, rects, bs, ranges, host.num_tiles);
+  launch_bin_gather(ctx->stream, 1, n, vals, rects, rect_sorted);
+  BinScratch bs{matrix, tile_tot, rect_sorted, run};
+  BinPass pass{0, n, nullptr, ranges, ebase, entries, true};
+  launch_bin_count(ctx->stream, dview, &host, 1, n, chunk, bs, pass);
   GSR_LAUNCHED(ctx);
   GSR_HIP(ctx, hipMemcpyAsync(ctx->pinned, entries, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
   GSR_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -285,8 +286,7 @@ int gsr_bin_sort(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, uint3
   if (E > order_capacity)
     return fail(ctx, GSR_E_CAPACITY, "order holds %llu entries, %llu needed", (unsigned long long)order_capacity,
                 (unsigned long long)E);
-  launch_bin_emit(ctx->stream, dview, 1, n, chunk, keys, vals, rects, bs, ranges, order, order_capacity, overflow,
-                  host.num_tiles);
+  launch_bin_emit(ctx->stream, dview, &host, 1, n, chunk, vals, bs, pass, order, order_capacity, overflow);
   GSR_LAUNCHED(ctx);
   return GSR_OK;
 }
@@ -304,6 +304,11 @@ static int raster_common(gsr_ctx* ctx, const gsr_camera* cam, RasterArgs& a, Vie
   a.queue = nullptr;
   a.tile_batch = 1;
   a.order_capacity = ~uint64_t(0);  // the caller's order holds every entry its ranges name
+  a.num_passes = 1;
+  a.ebase[0] = nullptr;
+  a.pass = 0;
+  a.tstate = nullptr;
+  a.done = nullptr;
   return GSR_OK;
 }
 
@@ -317,7 +322,7 @@ int gsr_forward(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, const 
   if (rc) return rc;
   a.splats = splats;
   a.order = order;
-  a.ranges = ranges;
+  a.ranges[0] = ranges;
   a.image = image;
   a.state = state;
   launch_forward(ctx->stream, a, 0, 0);
@@ -353,7 +358,7 @@ int gsr_backward(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, const
   if (rc) return rc;
   a.splats = splats;
   a.order = order;
-  a.ranges = ranges;
+  a.ranges[0] = ranges;
   a.image = const_cast<float*>(image);
   a.dl = loss_grad;
   a.grads = grads;

@@ -46,20 +46,28 @@ struct gsr_batch {
   float* splats = nullptr;
   uint32_t *keys = nullptr, *keys_alt = nullptr, *vals = nullptr, *vals_alt = nullptr;
   uint2* rects = nullptr;
+  uint2* rect_sorted = nullptr;
   uint32_t *hist = nullptr, *piece = nullptr;
   uint32_t* matrix = nullptr;
-  uint64_t* matrix_off = nullptr;
   uint32_t* tile_tot = nullptr;
-  uint64_t* entries = nullptr;
-  uint32_t* ranges = nullptr;
+  // Depth-slice passes: pass p bins sorted positions [pass_end[p-1], pass_end[p]).
+  uint32_t num_passes = 1;
+  uint32_t growth = 1;
+  uint32_t pass_end[kMaxPasses] = {};
+  uint64_t* entries = nullptr;  // [kMaxPasses][V]
+  uint64_t* ebase = nullptr;    // [kMaxPasses][V]
+  uint64_t* run = nullptr;      // slots used so far
+  uint32_t* ranges = nullptr;   // [kMaxPasses][total_tiles][2]
+  float* tstate = nullptr;      // per pixel transmittance between forward passes
+  uint8_t* done = nullptr;      // per tile: saturated
   uint32_t* order = nullptr;
   uint64_t order_cap = 0;
-  uint32_t* flags = nullptr;  // [0] overflow, [1] forward queue, [2] backward queue
+  uint32_t* flags = nullptr;  // [0] overflow, [1 + p] forward queue of pass p, [1 + kMaxPasses] backward queue
   float* image = nullptr;
   float* dl = nullptr;
   float* coef = nullptr;
   float* vgrads = nullptr;
-  uint64_t* pinned = nullptr;  // host: entries[V], overflow
+  uint64_t* pinned = nullptr;  // host: entries[kMaxPasses][V], overflow
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -128,11 +136,14 @@ int alloc_all(gsr_batch* b) {
       (rc = dalloc(b, &b->d_work, b->total_tiles)) || (rc = dalloc(b, &b->splats, vn * 12)) ||
       (rc = dalloc(b, &b->keys, vn)) || (rc = dalloc(b, &b->keys_alt, vn)) || (rc = dalloc(b, &b->vals, vn)) ||
       (rc = dalloc(b, &b->vals_alt, vn)) || (rc = dalloc(b, &b->rects, vn)) ||
+      (rc = dalloc(b, &b->rect_sorted, vn)) ||
       (rc = dalloc(b, &b->hist, std::max<size_t>(1, radix_hist_elems(b->n, b->V)))) ||
       (rc = dalloc(b, &b->piece,
                    std::max<size_t>(1, scan_piece_elems(radix_hist_elems(b->n, 1), b->V)))) ||
-      (rc = dalloc(b, &b->matrix_off, b->V)) || (rc = dalloc(b, &b->tile_tot, b->total_tiles)) ||
-      (rc = dalloc(b, &b->entries, b->V)) || (rc = dalloc(b, &b->ranges, size_t(b->total_tiles) * 2)) ||
+      (rc = dalloc(b, &b->tile_tot, b->total_tiles)) || (rc = dalloc(b, &b->entries, kMaxPasses * b->V)) ||
+      (rc = dalloc(b, &b->ebase, kMaxPasses * b->V)) || (rc = dalloc(b, &b->run, 1)) ||
+      (rc = dalloc(b, &b->ranges, size_t(kMaxPasses) * b->total_tiles * 2)) ||
+      (rc = dalloc(b, &b->tstate, b->total_pixels)) || (rc = dalloc(b, &b->done, b->total_tiles)) ||
       (rc = dalloc(b, &b->flags, 64)) || (rc = dalloc(b, &b->image, b->total_pixels * 3)) ||
       (rc = dalloc(b, &b->dl, b->total_pixels * 3)) ||
       (rc = dalloc(b, &b->vgrads, size_t(b->V) * std::max<uint32_t>(b->n, 1) * 16)))
@@ -146,17 +157,22 @@ int set_chunk(gsr_batch* b, uint32_t chunk) {
   const size_t m = bin_matrix_elems(b->views.data(), b->V, b->n, chunk);
   int rc = dalloc(b, &b->matrix, std::max<size_t>(1, m));
   if (rc) return rc;
-  std::vector<uint64_t> off(b->V);
-  const uint64_t chunks = (uint64_t(b->n) + chunk - 1) / chunk;
-  uint64_t run = 0;
-  for (uint32_t v = 0; v < b->V; ++v) {
-    off[v] = run;
-    run += chunks * b->views[v].num_tiles;
-  }
-  if (hipMemcpy(b->matrix_off, off.data(), off.size() * sizeof(uint64_t), hipMemcpyHostToDevice) != hipSuccess)
-    return ctx_fail(b->ctx, GSR_E_HIP, "upload of matrix offsets failed");
   drop_graph(b);
   return GSR_OK;
+}
+
+// passes == 1: full tile lists.  passes == k > 1: depth slices whose ends grow
+// by `growth`, the first holding ceil(n / growth^(k-1)) Gaussians.
+void set_passes(gsr_batch* b, uint32_t passes, uint32_t growth) {
+  b->num_passes = passes;
+  b->growth = growth;
+  uint64_t div = 1;
+  for (uint32_t p = 1; p < passes; ++p) div *= growth;
+  for (uint32_t p = 0; p < passes; ++p) {
+    b->pass_end[p] = p + 1 == passes ? b->n : uint32_t((uint64_t(b->n) + div - 1) / div);
+    div /= growth;
+  }
+  drop_graph(b);
 }
 
 int upload_cams(gsr_batch* b) {
@@ -171,8 +187,10 @@ int upload_cams(gsr_batch* b) {
 
 // Stage boundaries and workload counters of a profiled step.
 struct Prof {
-  hipEvent_t ev[GSR_STAGES + 1];
+  hipEvent_t ev[7];  // boundaries around project, sort, binning + forward, loss, backward, chain
+  float binned_ms[3];             // bin count, bin emit, forward (summed over passes)
   uint32_t* tile_last;
+  uint32_t* tile_walk;
   uint32_t* touched;
 };
 
@@ -185,18 +203,13 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   auto mark = [&]() {
     if (prof) hipEventRecord(prof->ev[stage++], s);
   };
-  launch_zero(s, b->flags, 16);
+  launch_zero(s, b->flags, 64 * sizeof(uint32_t));
   mark();
   launch_project(s, b->d_cams, b->d_views, b->V, gs, sh, b->deg, b->n, b->splats, b->keys, b->rects, nullptr);
   mark();
   SortScratch sc{b->keys_alt, b->vals, b->vals_alt, b->hist, b->piece};
   launch_depth_sort(s, b->keys, b->n, b->V, sc);
-  mark();
-  BinScratch bs{b->matrix, b->matrix_off, b->tile_tot, b->entries};
-  launch_bin_count(s, b->d_views, b->V, b->n, b->chunk, b->keys, b->vals, b->rects, bs, b->ranges, b->max_tiles);
-  mark();
-  launch_bin_emit(s, b->d_views, b->V, b->n, b->chunk, b->keys, b->vals, b->rects, bs, b->ranges, b->order,
-                  b->order_cap, b->flags, b->max_tiles);
+  launch_bin_gather(s, b->V, b->n, b->vals, b->rects, b->rect_sorted);
   mark();
   RasterArgs a{};
   a.views = b->d_views;
@@ -205,21 +218,70 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   a.splats = b->splats;
   a.order = b->order;
   a.order_capacity = b->order_cap;
-  a.ranges = b->ranges;
+  a.num_passes = int(b->num_passes);
+  for (uint32_t p = 0; p < b->num_passes; ++p) {
+    a.ranges[p] = b->ranges + size_t(p) * b->total_tiles * 2;
+    a.ebase[p] = b->ebase + size_t(p) * b->V;
+  }
   a.image = image;
   a.state = nullptr;
   a.dl = b->dl;
   a.grads = b->vgrads;
   a.grads_view_stride = uint64_t(b->n) * 16;
   a.tile_batch = b->tile_batch;
-  a.queue = b->flags + 1;
+  a.tstate = b->num_passes > 1 ? b->tstate : nullptr;
+  a.done = b->num_passes > 1 ? b->done : nullptr;
   a.tile_last = prof ? prof->tile_last : nullptr;
-  launch_forward(s, a, 1, b->grid_fwd);
+  a.tile_walk = prof ? prof->tile_walk : nullptr;
+  // Binning and forward alternate per depth slice: a slice is binned only for
+  // the tiles the slices before it left unsaturated.  Profiled steps time all
+  // binning as "bin count" + "bin emit" and all forward passes as "forward".
+  BinScratch bs{b->matrix, b->tile_tot, b->rect_sorted, b->run};
+  float t_count = 0.f, t_emit = 0.f, t_fwd = 0.f;
+  hipEvent_t e0, e1, e2, e3;
+  if (prof) {
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventCreate(&e2);
+    hipEventCreate(&e3);
+  }
+  for (uint32_t p = 0; p < b->num_passes; ++p) {
+    BinPass pass{p ? b->pass_end[p - 1] : 0u, b->pass_end[p], p ? b->done : nullptr,
+                 b->ranges + size_t(p) * b->total_tiles * 2, b->ebase + size_t(p) * b->V,
+                 b->entries + size_t(p) * b->V, p == 0};
+    if (prof) hipEventRecord(e0, s);
+    launch_bin_count(s, b->d_views, b->views.data(), b->V, b->n, b->chunk, bs, pass);
+    if (prof) hipEventRecord(e1, s);
+    launch_bin_emit(s, b->d_views, b->views.data(), b->V, b->n, b->chunk, b->vals, bs, pass, b->order, b->order_cap,
+                    b->flags);
+    if (prof) hipEventRecord(e2, s);
+    a.pass = int(p);
+    a.queue = b->flags + 1 + p;
+    launch_forward(s, a, 1, b->grid_fwd);
+    if (prof) {
+      hipEventRecord(e3, s);
+      hipEventSynchronize(e3);
+      float x;
+      hipEventElapsedTime(&x, e0, e1);
+      t_count += x;
+      hipEventElapsedTime(&x, e1, e2);
+      t_emit += x;
+      hipEventElapsedTime(&x, e2, e3);
+      t_fwd += x;
+    }
+  }
+  if (prof) {
+    prof->binned_ms[0] = t_count;
+    prof->binned_ms[1] = t_emit;
+    prof->binned_ms[2] = t_fwd;
+    for (hipEvent_t e : {e0, e1, e2, e3}) hipEventDestroy(e);
+  }
   a.tile_last = nullptr;
+  a.tile_walk = nullptr;
   mark();
   launch_loss_grad(s, b->d_views, b->views.data(), b->V, image, targets, lambda, b->coef, b->dl);
   mark();
-  a.queue = b->flags + 2;
+  a.queue = b->flags + 1 + kMaxPasses;
   launch_backward(s, a, 1, b->grid_bwd);
   mark();
   launch_shape(s, b->d_cams, b->V, gs, sh, b->deg, b->n, b->vgrads, uint64_t(b->n) * 16, shape, sh_grad, grads, 1,
@@ -255,8 +317,9 @@ int run_step(gsr_batch* b, const float* gs, const float* sh, const float* target
       issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, images);
       if (hipGetLastError() != hipSuccess) return ctx_fail(b->ctx, GSR_E_HIP, "kernel launch failed");
     }
-    if (hipMemcpyAsync(b->pinned, b->entries, b->V * sizeof(uint64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(b->pinned + b->V, b->flags, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess)
+    const size_t ne = size_t(b->num_passes) * b->V;
+    if (hipMemcpyAsync(b->pinned, b->entries, ne * sizeof(uint64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(b->pinned + ne, b->flags, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess)
       return ctx_fail(b->ctx, GSR_E_HIP, "readback failed");
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
@@ -264,8 +327,8 @@ int run_step(gsr_batch* b, const float* gs, const float* sh, const float* target
       return ctx_fail(b->ctx, GSR_E_HIP, m.c_str());
     }
     uint64_t total = 0;
-    for (uint32_t v = 0; v < b->V; ++v) total += b->pinned[v];
-    const bool overflow = (uint32_t(b->pinned[b->V]) & 0xFFFFFFFFu) != 0 || total > b->order_cap;
+    for (size_t k = 0; k < ne; ++k) total += b->pinned[k];
+    const bool overflow = (uint32_t(b->pinned[ne]) & 0xFFFFFFFFu) != 0 || total > b->order_cap;
     if (!overflow) return GSR_OK;
     // Grow the entry storage and run the step again.
     uint64_t cap = total + total / 4 + 1024;
@@ -304,7 +367,10 @@ int gsr_batch_create(gsr_ctx* ctx, uint32_t num_views, const gsr_camera* cams, u
   uint32_t chunk = 256;
   while (chunk < 4096 && uint64_t(chunk) * 2048 < items) chunk <<= 1;
   if (!rc) rc = set_chunk(b, chunk);
-  if (!rc && hipHostMalloc(reinterpret_cast<void**>(&b->pinned), (num_views + 8) * sizeof(uint64_t),
+  // Depth slices: most tiles of an opaque scene saturate within the nearest
+  // eighth of the Gaussians, and the rest rarely saturate at all.
+  if (!rc) set_passes(b, n >= 65536 ? 2 : 1, n >= 65536 ? 8 : 1);
+  if (!rc && hipHostMalloc(reinterpret_cast<void**>(&b->pinned), (kMaxPasses * num_views + 8) * sizeof(uint64_t),
                            hipHostMallocDefault) != hipSuccess)
     rc = ctx_fail(ctx, GSR_E_HIP, "hipHostMalloc failed");
   if (!rc) rc = upload_cams(b);
@@ -332,8 +398,8 @@ void gsr_batch_destroy(gsr_batch* b) {
   hipStreamSynchronize(ctx_stream(b->ctx));
   drop_graph(b);
   void* ptrs[] = {b->d_cams, b->d_views, b->d_work, b->splats, b->keys, b->keys_alt, b->vals, b->vals_alt,
-                  b->rects, b->hist, b->piece, b->matrix, b->matrix_off, b->tile_tot, b->entries, b->ranges,
-                  b->order, b->flags, b->image, b->dl, b->coef, b->vgrads};
+                  b->rects, b->rect_sorted, b->hist, b->piece, b->matrix, b->tile_tot, b->entries, b->ebase,
+                  b->run, b->ranges, b->tstate, b->done, b->order, b->flags, b->image, b->dl, b->coef, b->vgrads};
   for (void* p : ptrs) dfree(p);
   if (b->pinned) hipHostFree(b->pinned);
   delete b;
@@ -359,7 +425,22 @@ int gsr_batch_step(gsr_batch* b, const float* gs, const float* sh, const float* 
 
 int gsr_batch_entries(gsr_batch* b, uint64_t* entries) {
   if (!b || !entries) return GSR_E_INVALID;
-  for (uint32_t v = 0; v < b->V; ++v) entries[v] = b->pinned[v];
+  for (uint32_t v = 0; v < b->V; ++v) {
+    entries[v] = 0;
+    for (uint32_t p = 0; p < b->num_passes; ++p) entries[v] += b->pinned[size_t(p) * b->V + v];
+  }
+  return GSR_OK;
+}
+
+int gsr_batch_set_passes(gsr_batch* b, uint32_t passes, uint32_t growth) {
+  if (!b) return GSR_E_INVALID;
+  if (passes < 1 || passes > uint32_t(kMaxPasses)) return ctx_fail(b->ctx, GSR_E_INVALID, "passes not in [1, 8]");
+  if (passes > 1 && (growth < 2 || growth > 16)) return ctx_fail(b->ctx, GSR_E_INVALID, "growth not in [2, 16]");
+  if (passes == 1) growth = 1;
+  uint64_t div = 1;
+  for (uint32_t p = 1; p < passes; ++p) div *= growth;
+  if (div > b->n) return ctx_fail(b->ctx, GSR_E_INVALID, "too few Gaussians for the passes");
+  if (passes != b->num_passes || growth != b->growth) set_passes(b, passes, growth);
   return GSR_OK;
 }
 
@@ -385,10 +466,13 @@ int gsr_batch_profile(gsr_batch* b, const float* gs, const float* sh, const floa
   if (rc) return rc;
   hipStream_t s = ctx_stream(b->ctx);
   Prof prof{};
-  std::vector<uint32_t> tl(b->total_tiles), touched(b->V);
-  if (hipMalloc(reinterpret_cast<void**>(&prof.tile_last), std::max<size_t>(4, tl.size() * 4)) != hipSuccess ||
+  const size_t nt = std::max<size_t>(1, b->total_tiles);
+  std::vector<uint32_t> tl(nt), tw(nt), touched(b->V);
+  if (hipMalloc(reinterpret_cast<void**>(&prof.tile_last), nt * 4) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&prof.tile_walk), nt * 4) != hipSuccess ||
       hipMalloc(reinterpret_cast<void**>(&prof.touched), b->V * 4) != hipSuccess) {
     dfree(prof.tile_last);
+    dfree(prof.tile_walk);
     return ctx_fail(b->ctx, GSR_E_NOMEM, "profile buffers");
   }
   launch_zero(s, prof.touched, b->V * 4);
@@ -396,23 +480,39 @@ int gsr_batch_profile(gsr_batch* b, const float* gs, const float* sh, const floa
   issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, &prof);
   hipError_t e = hipStreamSynchronize(s);
   if (e == hipSuccess) {
-    for (int k = 0; k < GSR_STAGES; ++k) hipEventElapsedTime(stage_ms + k, prof.ev[k], prof.ev[k + 1]);
+    // Stage order: project, depth sort, bin count, bin emit, forward, loss, backward, chain.
+    hipEventElapsedTime(stage_ms + 0, prof.ev[0], prof.ev[1]);
+    hipEventElapsedTime(stage_ms + 1, prof.ev[1], prof.ev[2]);
+    stage_ms[2] = prof.binned_ms[0];
+    stage_ms[3] = prof.binned_ms[1];
+    stage_ms[4] = prof.binned_ms[2];
+    hipEventElapsedTime(stage_ms + 5, prof.ev[3], prof.ev[4]);
+    hipEventElapsedTime(stage_ms + 6, prof.ev[4], prof.ev[5]);
+    hipEventElapsedTime(stage_ms + 7, prof.ev[5], prof.ev[6]);
     if (counters) {
-      hipMemcpy(tl.data(), prof.tile_last, tl.size() * 4, hipMemcpyDeviceToHost);
+      hipMemcpy(tl.data(), prof.tile_last, nt * 4, hipMemcpyDeviceToHost);
+      hipMemcpy(tw.data(), prof.tile_walk, nt * 4, hipMemcpyDeviceToHost);
       hipMemcpy(touched.data(), prof.touched, b->V * 4, hipMemcpyDeviceToHost);
-      std::vector<uint64_t> ent(b->V);
-      hipMemcpy(ent.data(), b->entries, b->V * 8, hipMemcpyDeviceToHost);
+      std::vector<uint64_t> pe(size_t(b->num_passes) * b->V), ent(b->V, 0);
+      hipMemcpy(pe.data(), b->entries, pe.size() * 8, hipMemcpyDeviceToHost);
+      for (uint32_t p = 0; p < b->num_passes; ++p)
+        for (uint32_t v = 0; v < b->V; ++v) ent[v] += pe[size_t(p) * b->V + v];
       for (uint32_t v = 0; v < b->V; ++v) {
-        uint64_t f = 0;
-        for (uint32_t t = 0; t < b->views[v].num_tiles; ++t) f += tl[b->views[v].tile_base + t];
-        counters[3 * v] = ent[v];
-        counters[3 * v + 1] = f;
-        counters[3 * v + 2] = touched[v];
+        uint64_t f = 0, w = 0;
+        for (uint32_t t = 0; t < b->views[v].num_tiles; ++t) {
+          f += tl[b->views[v].tile_base + t];
+          w += tw[b->views[v].tile_base + t];
+        }
+        counters[4 * v] = ent[v];
+        counters[4 * v + 1] = f;
+        counters[4 * v + 2] = touched[v];
+        counters[4 * v + 3] = w;
       }
     }
   }
   for (auto& ev : prof.ev) hipEventDestroy(ev);
   dfree(prof.tile_last);
+  dfree(prof.tile_walk);
   dfree(prof.touched);
   if (e != hipSuccess) return ctx_fail(b->ctx, GSR_E_HIP, "profiled step failed");
   return GSR_OK;

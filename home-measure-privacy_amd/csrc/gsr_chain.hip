@@ -56,7 +56,11 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
       const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
       r[0] = z; r[1] = z; r[2] = z;
     }
-    if (touched && any) atomicAdd(touched + v, 1u);  // profiling only
+    if (touched) {  // profiling only: one atomic per wave
+      const uint64_t m = __ballot(any);
+      if (m && __lane_id() == uint32_t(__ffsll(static_cast<unsigned long long>(m)) - 1))
+        atomicAdd(touched + v, uint32_t(__popcll(m)));
+    }
     if (!screen && !colour) continue;
     const CamF cam = load_cam(cams[v]);
     float p[3];

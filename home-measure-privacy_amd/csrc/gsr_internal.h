@@ -32,7 +32,6 @@ struct ViewDesc {
   uint32_t num_tiles;
   uint64_t pixel_base;  // first pixel of this view in concatenated images
   uint64_t splat_base;  // view * n
-  uint64_t entry_base;  // first slot of this view in the concatenated order
 };
 
 // One unit of blend work: a 16x16 tile of one view.
@@ -67,42 +66,64 @@ size_t scan_piece_elems(size_t seg_len, uint32_t num_segs);
 void launch_depth_sort(hipStream_t s, uint32_t* keys, uint32_t n, uint32_t num_views, const SortScratch& sc);
 
 struct BinScratch {
-  uint32_t* matrix;      // [sum_v chunks_v * T_v]
-  uint64_t* matrix_off;  // [V] offset of each view's matrix
+  uint32_t* matrix;      // counts of one pass: [view][chunk][tile] at chunks * tile_base
   uint32_t* tile_tot;    // [total tiles]
-  uint64_t* entries;     // [V] entry count per view
+  uint2* rect_sorted;    // [V][n] tile rectangles in depth order
+  uint64_t* run;         // device scalar: order slots used by the passes so far
 };
-// Pass 1 of binning: per-(chunk, tile) counts, tile totals, ranges (view-local
-// entry positions), entry counts and views[v].entry_base.  keys/vals: the
-// depth-sorted (key, Gaussian) pairs of each view.
-void launch_bin_count(hipStream_t s, const ViewDesc* views, uint32_t num_views, uint32_t n, uint32_t chunk,
-                      const uint32_t* keys, const uint32_t* vals, const uint2* rects, const BinScratch& bs,
-                      uint32_t* ranges, uint32_t max_tiles_per_view);
-// Pass 2: the order itself.  Writes beyond order_capacity are dropped and set
-// *overflow (device) nonzero.
-void launch_bin_emit(hipStream_t s, const ViewDesc* views, uint32_t num_views, uint32_t n, uint32_t chunk,
-                     const uint32_t* keys, const uint32_t* vals, const uint2* rects, const BinScratch& bs,
-                     const uint32_t* ranges, uint32_t* order, uint64_t order_capacity, uint32_t* overflow,
-                     uint32_t max_tiles_per_view);
+// One binning pass: the depth slice [pbeg, pend) of every view's sorted
+// Gaussians.  Its entries go to order[ebase[v] + ranges[...]].
+struct BinPass {
+  uint32_t pbeg, pend;
+  const uint8_t* done;  // per global tile, nonzero: saturated, gets no entries (may be null)
+  uint32_t* ranges;     // [total tiles][2], view-local positions within the pass
+  uint64_t* ebase;      // [V] first order slot of each view's entries in this pass
+  uint64_t* entries;    // [V] entries per view in this pass
+  bool first;           // first pass of the step: order slots start at 0
+};
+// rects in depth order (once per step, before the passes).
+void launch_bin_gather(hipStream_t s, uint32_t num_views, uint32_t n, const uint32_t* vals, const uint2* rects,
+                       uint2* rect_sorted);
+// Pass part 1: per-(chunk, tile) counts, tile totals, ranges, entry counts and
+// entry bases.  host_views: the views on the host (grid sizing).
+void launch_bin_count(hipStream_t s, const ViewDesc* views, const ViewDesc* host_views, uint32_t num_views,
+                      uint32_t n, uint32_t chunk, const BinScratch& bs, const BinPass& pass);
+// Pass part 2: the order itself.  Writes beyond order_capacity are dropped and
+// set *overflow (device) nonzero.
+void launch_bin_emit(hipStream_t s, const ViewDesc* views, const ViewDesc* host_views, uint32_t num_views,
+                     uint32_t n, uint32_t chunk, const uint32_t* vals, const BinScratch& bs, const BinPass& pass,
+                     uint32_t* order, uint64_t order_capacity, uint32_t* overflow);
 size_t bin_matrix_elems(const ViewDesc* host_views, uint32_t num_views, uint32_t n, uint32_t chunk);
 
-// Blending (gsr_raster.hip).
+// Blending (gsr_raster.hip).  A tile's list may come in several passes (depth
+// slices); pass p's entries of tile g are order[ebase[p][view] + ranges[p][2g ..
+// 2g+1]], and the passes follow each other in depth order.
+constexpr int kMaxPasses = 8;
 struct RasterArgs {
   const ViewDesc* views;
   const TileWork* work;  // [num_work]
   uint32_t num_work;
   const float* splats;
   const uint32_t* order;
-  uint64_t order_capacity;  // slots in order; tiles whose range ends beyond it are skipped
-  const uint32_t* ranges;
+  uint64_t order_capacity;  // slots in order; segments ending beyond it are skipped
+  int num_passes;
+  const uint32_t* ranges[kMaxPasses];
+  const uint64_t* ebase[kMaxPasses];  // null: 0 for every view
   float* image;          // concatenated views
-  float* state;          // optional
+  float* state;          // optional (T, last), single pass only
   const float* dl;       // backward: dLoss/dPixel, concatenated views
   float* grads;          // backward: [V][n][16]
   uint64_t grads_view_stride;  // elements between views' gradient records (0: shared)
   uint32_t* queue;       // persistent mode: work counter (zeroed before launch)
   uint32_t tile_batch;   // tiles claimed per queue pop
-  uint32_t* tile_last;   // forward, optional: per global tile, furthest last hit - start
+  // Forward over passes: pass blends segment `pass`, resuming from tstate (per
+  // pixel transmittance) and image; done[g] is set once all of tile g's pixels
+  // have T <= 1e-4, and later passes skip the tile.
+  int pass;
+  float* tstate;         // null for a single pass
+  uint8_t* done;         // null for a single pass
+  uint32_t* tile_last;   // forward, optional: per global tile, furthest last hit - start (over passes)
+  uint32_t* tile_walk;   // forward, optional: per global tile, entries examined (over passes)
 };
 void launch_forward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid);
 void launch_backward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid);

@@ -44,15 +44,22 @@ __device__ inline Tile load_tile(const RasterArgs& a, uint32_t w) {
   t.width = vd.width;
   t.height = vd.height;
   t.gtile = vd.tile_base + ty * vd.tiles_x + tx;
-  t.start = a.ranges[size_t(t.gtile) * 2];
-  t.end = a.ranges[size_t(t.gtile) * 2 + 1];
-  // A batched step whose entries overflowed the order storage is rerun; its
-  // ranges then point past the storage, so such tiles read nothing.
-  if (t.end < t.start || vd.entry_base + t.end > a.order_capacity) t.start = t.end = 0;
-  t.order = a.order + vd.entry_base;
+  t.start = t.end = 0;
+  t.order = a.order;
   t.splats = a.splats + vd.splat_base * 12;
   t.pixel_base = vd.pixel_base;
   return t;
+}
+
+// Points t at its entries of pass p.  A batched step whose entries overflowed the
+// order storage is rerun; its ranges then point past the storage, so such
+// segments read nothing.
+__device__ inline void tile_segment(const RasterArgs& a, int p, Tile& t) {
+  const uint64_t eb = a.ebase[p] ? a.ebase[p][t.view] : 0;
+  t.start = a.ranges[p][size_t(t.gtile) * 2];
+  t.end = a.ranges[p][size_t(t.gtile) * 2 + 1];
+  if (t.end < t.start || eb + t.end > a.order_capacity) t.start = t.end = 0;
+  t.order = a.order + eb;
 }
 
 // Records of entries [k0, k0 + 64): lane j fetches entry k0 + j.
@@ -78,14 +85,25 @@ __device__ inline Batch fetch_batch(const Tile& t, uint32_t k0, uint32_t lane) {
   return b;
 }
 
-// alphaAt (gpu.js:92-103).  Returns alpha, or 0 for a miss.
-__device__ inline float alpha_at(float dx, float dy, float A, float B, float C, float op, float rad) {
-  if (fabsf(dx) > rad || fabsf(dy) > rad) return 0.0f;
-  float power = -0.5f * (A * dx * dx + C * dy * dy) - B * dx * dy;
-  if (power > 0.0f) return 0.0f;
-  float alpha = fminf(kAlphaMax, op * __expf(power));
-  if (alpha < kAlphaMin) return 0.0f;
-  return alpha;
+// Per-entry conic coefficients of the exponent in base 2:
+// power * log2(e) = (qA dx^2 + qC dy^2) - qB dx dy with qA = -A/2 log2 e,
+// qB = B log2 e, qC = -C/2 log2 e.  The sign (the power > 0 test) is unchanged.
+struct Conic2 {
+  float qA, qB, qC;
+};
+
+__device__ inline Conic2 conic2(float A, float B, float C) {
+  const float l2e = 1.4426950408889634f;
+  return Conic2{-0.5f * l2e * A, l2e * B, -0.5f * l2e * C};
+}
+
+// alphaAt (gpu.js:92-103) without branches: alpha, or 0 for a miss.  The miss
+// tests are the reference's, negated as written there (a NaN takes the same path).
+__device__ inline float alpha_at(float dx, float dy, const Conic2& q, float op, float rad) {
+  const float p2 = (q.qA * dx * dx + q.qC * dy * dy) - q.qB * dx * dy;
+  const float alpha = fminf(kAlphaMax, op * __builtin_amdgcn_exp2f(p2));
+  const bool miss = fabsf(dx) > rad || fabsf(dy) > rad || p2 > 0.0f || alpha < kAlphaMin;
+  return miss ? 0.0f : alpha;
 }
 
 __device__ inline bool any_active(const float T[kPix]) {
@@ -95,11 +113,49 @@ __device__ inline bool any_active(const float T[kPix]) {
   return __ballot(act) != 0;
 }
 
+// Pixel-centre bounding box of the wave's still-active pixels (empty: x0 > x1).
+struct Box {
+  float x0, x1, y0, y1;
+};
+
+__device__ inline Box active_box(float px, const float py[kPix], const float T[kPix]) {
+  float ylo = __builtin_huge_valf(), yhi = -__builtin_huge_valf();
+#pragma unroll
+  for (int i = 0; i < kPix; ++i)
+    if (T[i] > kTStop) {
+      ylo = fminf(ylo, py[i]);
+      yhi = fmaxf(yhi, py[i]);
+    }
+  const bool any = ylo <= yhi;
+  Box b{any ? px : __builtin_huge_valf(), any ? px : -__builtin_huge_valf(), ylo, yhi};
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    b.x0 = fminf(b.x0, __shfl_xor(b.x0, off));
+    b.x1 = fmaxf(b.x1, __shfl_xor(b.x1, off));
+    b.y0 = fminf(b.y0, __shfl_xor(b.y0, off));
+    b.y1 = fmaxf(b.y1, __shfl_xor(b.y1, off));
+  }
+  return b;
+}
+
+// True when the splat's square footprint (|dx| <= r and |dy| <= r, alphaAt's
+// cull) misses every active pixel.  Uses alphaAt's own subtraction px - u, which
+// is monotonic in px, so the test is exact at the box edges.
+__device__ inline bool box_culled(const Box& b, float u, float v, float r) {
+  return (b.x1 - u) < -r || (b.x0 - u) > r || (b.y1 - v) < -r || (b.y0 - v) > r;
+}
+
 // ---- forward -------------------------------------------------------------------
 
+// kLast: also track each pixel's last hit (the optional state output and the
+// profiling counters need it; the batched training step does not).
+template <bool kLast>
 __device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) {
   const uint32_t lane = __lane_id();
-  const Tile t = load_tile(a, w);
+  Tile t = load_tile(a, w);
+  const bool resume = a.pass > 0;
+  if (resume && a.done[t.gtile]) return;  // saturated by an earlier pass
+  tile_segment(a, a.pass, t);
   const uint32_t x = t.x0 + (lane & 15);
   const float px = float(x) + 0.5f;
   float py[kPix], T[kPix], cr[kPix], cg[kPix], cb[kPix];
@@ -112,8 +168,16 @@ __device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) 
     T[i] = inside ? 1.0f : 0.0f;  // outside pixels count as done
     cr[i] = cg[i] = cb[i] = 0.0f;
     last[i] = t.start;
+    if (resume && inside) {
+      size_t pix = t.pixel_base + size_t(y) * t.width + x;
+      T[i] = a.tstate[pix];
+      cr[i] = a.image[pix * 3];
+      cg[i] = a.image[pix * 3 + 1];
+      cb[i] = a.image[pix * 3 + 2];
+    }
   }
   Batch nb = fetch_batch(t, t.start, lane);
+  uint32_t walked = t.start;
   for (uint32_t k0 = t.start; k0 < t.end; k0 += 64) {
     if (!any_active(T)) break;
     __syncthreads();
@@ -123,32 +187,49 @@ __device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) 
     __syncthreads();
     if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
     const uint32_t nk = min(64u, t.end - k0);
+    Box box = active_box(px, py, T);
     for (uint32_t j = 0; j < nk; ++j) {
-      if ((j & 7) == 0 && j != 0 && !any_active(T)) break;
-      const float4 r0 = lds[j][0], r1 = lds[j][1], r2 = lds[j][2];
+      if ((j & 15) == 0 && j != 0) {
+        if (!any_active(T)) break;
+        if ((j & 31) == 0) box = active_box(px, py, T);  // a stale box is a superset: still exact
+      }
+      walked = k0 + j + 1;
+      const float4 r0 = lds[j][0], r1 = lds[j][1];
+      if (box_culled(box, r0.x, r0.y, r1.z)) continue;
+      const float4 r2 = lds[j][2];
       if (r2.w < 0.5f) continue;
+      const Conic2 q = conic2(r0.z, r0.w, r1.x);
 #pragma unroll
       for (int i = 0; i < kPix; ++i) {
-        if (!(T[i] > kTStop)) continue;
-        float dx = px - r0.x, dy = py[i] - r0.y;
-        float alpha = alpha_at(dx, dy, r0.z, r0.w, r1.x, r1.y, r1.z);
-        if (alpha == 0.0f) continue;
-        float wgt = alpha * T[i];
+        // A miss (or a done pixel) blends with weight 0: colour and T unchanged.
+        float alpha = alpha_at(px - r0.x, py[i] - r0.y, q, r1.y, r1.z);
+        alpha = T[i] > kTStop ? alpha : 0.0f;
+        const float wgt = alpha * T[i];
         cr[i] = fmaf(r2.x, wgt, cr[i]);
         cg[i] = fmaf(r2.y, wgt, cg[i]);
         cb[i] = fmaf(r2.z, wgt, cb[i]);
         T[i] = T[i] * (1.0f - alpha);
-        last[i] = k0 + j + 1;
+        if (kLast) last[i] = alpha != 0.0f ? k0 + j + 1 : last[i];
       }
     }
   }
-  if (a.tile_last) {
-    // Profiling: the tile's saturated prefix, furthest last hit minus start.
+  if (kLast && a.tile_last) {
+    // Profiling over passes: the tile's saturated prefix (furthest last hit,
+    // counted from the start of its first pass) and the entries examined.
     uint32_t m = t.start;
 #pragma unroll
     for (int i = 0; i < kPix; ++i) m = max(m, last[i]);
     for (int off = 32; off > 0; off >>= 1) m = max(m, uint32_t(__shfl_xor(int(m), off)));
-    if (lane == 0) a.tile_last[t.gtile] = m - t.start;
+    if (lane == 0) {
+      const uint32_t before = resume ? a.tile_walk[t.gtile] : 0u;
+      if (!resume) a.tile_last[t.gtile] = 0;
+      if (m > t.start) a.tile_last[t.gtile] = before + (m - t.start);
+      a.tile_walk[t.gtile] = before + (walked - t.start);
+    }
+  }
+  if (a.done) {
+    const bool open = any_active(T);
+    if (lane == 0) a.done[t.gtile] = open ? 0 : 1;
   }
 #pragma unroll
   for (int i = 0; i < kPix; ++i) {
@@ -158,16 +239,22 @@ __device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) 
     a.image[pix * 3] = cr[i];
     a.image[pix * 3 + 1] = cg[i];
     a.image[pix * 3 + 2] = cb[i];
-    if (a.state) {
+    if (a.tstate) a.tstate[pix] = T[i];
+    if (kLast && a.state) {
       a.state[pix * 2] = T[i];
       a.state[pix * 2 + 1] = float(last[i]);
     }
   }
 }
 
+__device__ inline void forward_any(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) {
+  if (a.state || a.tile_last) forward_tile<true>(a, w, lds);
+  else forward_tile<false>(a, w, lds);
+}
+
 __global__ __launch_bounds__(64) void forward_kernel(RasterArgs a) {
   __shared__ float4 lds[64][3];
-  forward_tile(a, blockIdx.x, lds);
+  forward_any(a, blockIdx.x, lds);
 }
 
 __device__ inline uint32_t pop_work(uint32_t* queue, uint32_t n) {
@@ -182,7 +269,7 @@ __global__ __launch_bounds__(64) void forward_persistent_kernel(RasterArgs a) {
     uint32_t w0 = pop_work(a.queue, a.tile_batch);
     if (w0 >= a.num_work) break;
     uint32_t w1 = min(a.num_work, w0 + a.tile_batch);
-    for (uint32_t w = w0; w < w1; ++w) forward_tile(a, w, lds);
+    for (uint32_t w = w0; w < w1; ++w) forward_any(a, w, lds);
   }
 }
 
@@ -208,7 +295,7 @@ __device__ inline float lane63(float v) { return __int_as_float(__builtin_amdgcn
 
 __device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3], uint32_t* gi_lds) {
   const uint32_t lane = __lane_id();
-  const Tile t = load_tile(a, w);
+  Tile t = load_tile(a, w);
   const uint32_t x = t.x0 + (lane & 15);
   const float px = float(x) + 0.5f;
   float py[kPix], T[kPix], ar[kPix], ag[kPix], ab[kPix];
@@ -232,6 +319,9 @@ __device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3],
     }
   }
   float* grads = a.grads + size_t(t.view) * a.grads_view_stride;
+  // The passes of a tile follow each other in depth order; pixel state carries over.
+  for (int p = 0; p < a.num_passes; ++p) {
+  tile_segment(a, p, t);
   Batch nb = fetch_batch(t, t.start, lane);
   for (uint32_t k0 = t.start; k0 < t.end; k0 += 64) {
     if (!any_active(T)) break;
@@ -243,24 +333,33 @@ __device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3],
     __syncthreads();
     if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
     const uint32_t nk = min(64u, t.end - k0);
+    Box box = active_box(px, py, T);
     for (uint32_t j = 0; j < nk; ++j) {
-      if ((j & 7) == 0 && j != 0 && !any_active(T)) break;
-      const float4 r0 = lds[j][0], r1 = lds[j][1], r2 = lds[j][2];
+      if ((j & 15) == 0 && j != 0) {
+        if (!any_active(T)) break;
+        box = active_box(px, py, T);
+      }
+      const float4 r0 = lds[j][0], r1 = lds[j][1];
+      if (box_culled(box, r0.x, r0.y, r1.z)) continue;
+      const float4 r2 = lds[j][2];
       if (r2.w < 0.5f) continue;
       const float A = r0.z, B = r0.w, C = r1.x, op = r1.y;
+      const Conic2 q = conic2(A, B, C);
+      const float omo = 1.0f - op;
       float g[9];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) g[q] = 0.0f;
+      for (int m = 0; m < 9; ++m) g[m] = 0.0f;
       bool hit = false;
 #pragma unroll
       for (int i = 0; i < kPix; ++i) {
-        if (!(T[i] > kTStop)) continue;
-        float dx = px - r0.x, dy = py[i] - r0.y;
-        float alpha = alpha_at(dx, dy, A, B, C, op, r1.z);
-        if (alpha == 0.0f) continue;
-        hit = true;
-        // Same arithmetic as forward_tile: acc and T follow it bit for bit.
-        float wgt = alpha * T[i];
+        const float dx = px - r0.x, dy = py[i] - r0.y;
+        float alpha = alpha_at(dx, dy, q, op, r1.z);
+        alpha = T[i] > kTStop ? alpha : 0.0f;
+        const bool h = alpha != 0.0f;
+        hit |= h;
+        // Same arithmetic as forward_tile: acc and T follow it bit for bit (a
+        // miss has weight 0 and leaves both unchanged).
+        const float wgt = alpha * T[i];
         ar[i] = fmaf(r2.x, wgt, ar[i]);
         ag[i] = fmaf(r2.y, wgt, ag[i]);
         ab[i] = fmaf(r2.z, wgt, ab[i]);
@@ -271,32 +370,33 @@ __device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3],
         g[0] += dr[i] * wgt;
         g[1] += dg[i] * wgt;
         g[2] += db[i] * wgt;
-        float inv = 1.0f / oma;
+        const float inv = __builtin_amdgcn_rcpf(oma);
         float dA = dr[i] * (Tk * r2.x - (fr[i] - ar[i]) * inv) + dg[i] * (Tk * r2.y - (fg[i] - ag[i]) * inv) +
                    db[i] * (Tk * r2.z - (fb[i] - ab[i]) * inv);
-        if (alpha < kAlphaMax) {
-          // gpu.js:270-281: opacity logit and screen (u, v, conic) gradients.
-          g[3] += dA * alpha * (1.0f - op);
-          float dP = dA * alpha;
-          g[4] += dP * (A * dx + B * dy);
-          g[5] += dP * (C * dy + B * dx);
-          g[6] += dP * -0.5f * dx * dx;
-          g[7] += dP * -dx * dy;
-          g[8] += dP * -0.5f * dy * dy;
-        }
+        // gpu.js:270-281: opacity logit and screen (u, v, conic) gradients, only
+        // while the 0.99 clamp is not active.
+        dA = (h && alpha < kAlphaMax) ? dA : 0.0f;
+        g[3] += dA * alpha * omo;
+        const float dP = dA * alpha;
+        g[4] += dP * (A * dx + B * dy);
+        g[5] += dP * (C * dy + B * dx);
+        g[6] += dP * -0.5f * dx * dx;
+        g[7] += dP * -dx * dy;
+        g[8] += dP * -0.5f * dy * dy;
       }
       if (__ballot(hit) == 0) continue;
       float s[9];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) s[q] = lane63(wave_sum63(g[q]));
+      for (int m = 0; m < 9; ++m) s[m] = lane63(wave_sum63(g[m]));
       float val = 0.0f;
 #pragma unroll
-      for (int q = 0; q < 9; ++q) val = lane == uint32_t(q) ? s[q] : val;
+      for (int m = 0; m < 9; ++m) val = lane == uint32_t(m) ? s[m] : val;
       if (lane < 9 && val != 0.0f) {
         const uint32_t gi = gi_lds[j];
         unsafeAtomicAdd(grads + size_t(gi) * 16 + lane, val);
       }
     }
+  }
   }
 }
 

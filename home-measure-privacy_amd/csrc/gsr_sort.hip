@@ -198,81 +198,108 @@ void scan_segments(hipStream_t st, uint32_t* data, size_t seg_len, uint32_t segs
 }
 
 // ---- binning -------------------------------------------------------------------
-// Tile counters live in LDS, one band of at most kBand tiles at a time.
-constexpr uint32_t kBand = 8192;
+// Blocks work on (chunk of depth-sorted Gaussians, band of tile rows).  A band
+// holds about kBandTiles tiles, so its counters fit a few KB of LDS and many
+// blocks stay resident per CU.  A pass bins one depth slice [pbeg, pend) of every
+// view's sorted Gaussians; tiles flagged done (already saturated by the slices
+// before) receive no entries.  The count matrix of a pass is laid out
+// [view][chunk][tile] at offset chunks * tile_base of the view.
+constexpr uint32_t kBandTiles = 2048;
 
-// Tile index of the t-th tile of a rectangle w tiles wide starting at (x0, y0).
-__device__ inline uint32_t rect_tile(uint32_t t, uint32_t w, float rcp_w, uint32_t x0, uint32_t y0, uint32_t tiles_x) {
-  uint32_t row = uint32_t(float(t) * rcp_w);
-  while (row * w > t) --row;
-  while ((row + 1) * w <= t) ++row;
-  uint32_t col = t - row * w;
-  return (y0 + row) * tiles_x + x0 + col;
+__host__ __device__ inline uint32_t band_rows(uint32_t tiles_x, uint32_t tiles_y) {
+  uint32_t r = kBandTiles / (tiles_x ? tiles_x : 1);
+  r = r < 1 ? 1 : r;
+  return r < tiles_y ? r : (tiles_y ? tiles_y : 1);
 }
 
-struct RectInfo {
-  uint32_t x0, y0, w, cnt;
-  uint32_t gi;
+struct Rect {
+  uint32_t x0, x1, y0, y1;  // inclusive; empty when x0 > x1 or y0 > y1
 };
 
-__device__ inline RectInfo rect_info(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                     const uint2* __restrict__ rects, size_t seg, uint32_t pos, uint32_t end) {
-  RectInfo r{0u, 0u, 1u, 0u, 0u};
-  if (pos < end && keys[seg + pos] != kInvalidKey) {
-    r.gi = vals[seg + pos];
-    uint2 rc = rects[seg + r.gi];
-    uint32_t x0 = rc.x & 0xFFFFu, x1 = rc.x >> 16, y0 = rc.y & 0xFFFFu, y1 = rc.y >> 16;
-    if (x0 <= x1 && y0 <= y1) {
-      r.x0 = x0; r.y0 = y0; r.w = x1 - x0 + 1; r.cnt = r.w * (y1 - y0 + 1);
-    }
-  }
-  return r;
+__device__ inline Rect unpack_rect(uint2 rc) {
+  return Rect{rc.x & 0xFFFFu, rc.x >> 16, rc.y & 0xFFFFu, rc.y >> 16};
 }
 
-// Pass 1: per (chunk, tile) entry counts into matrix[v][chunk][tile].
-// One wave per block walks its chunk's Gaussians in depth order; for each
-// Gaussian the 64 lanes spread over its tiles.
+// rects in depth order: sorted[v][p] = rects[v][vals[v][p]] (empty rectangles
+// mark rejected Gaussians), so band blocks read them coalesced.
+__global__ __launch_bounds__(256) void bin_gather_kernel(const uint32_t* __restrict__ vals,
+                                                         const uint2* __restrict__ rects, uint32_t n,
+                                                         uint2* __restrict__ sorted) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t seg = size_t(blockIdx.y) * n;
+  if (p < n) sorted[seg + p] = rects[seg + vals[seg + p]];
+}
+
+struct Band {
+  uint32_t tx, r0, r1;  // tiles per row, band rows [r0, r1)
+  bool valid;
+};
+
+__device__ inline Band band_of(const ViewDesc& vd, uint32_t band) {
+  const uint32_t R = band_rows(vd.tiles_x, vd.tiles_y);
+  Band b;
+  b.tx = vd.tiles_x;
+  b.r0 = band * R;
+  b.valid = b.r0 < vd.tiles_y;
+  b.r1 = min(vd.tiles_y, b.r0 + R);
+  return b;
+}
+
+// Pass 1: per (chunk, tile) entry counts.  Each Gaussian adds its rectangle
+// (clipped to the band) to a 2D difference array with four LDS atomics; prefix
+// sums along x then y turn it into counts.  Done tiles count zero.
 __global__ __launch_bounds__(64) void bin_count_kernel(const ViewDesc* __restrict__ views, uint32_t n, uint32_t chunk,
-                                                       uint32_t chunks, const uint32_t* __restrict__ keys,
-                                                       const uint32_t* __restrict__ vals,
-                                                       const uint2* __restrict__ rects,
-                                                       const uint64_t* __restrict__ matrix_off,
+                                                       uint32_t pbeg, uint32_t pend, uint32_t chunks,
+                                                       const uint2* __restrict__ sorted,
+                                                       const uint8_t* __restrict__ done,
                                                        uint32_t* __restrict__ matrix) {
-  extern __shared__ uint32_t cnt[];
-  const uint32_t v = blockIdx.y, c = blockIdx.x, lane = threadIdx.x;
+  extern __shared__ int diff[];
+  const uint32_t v = blockIdx.z, c = blockIdx.x, lane = threadIdx.x;
   const ViewDesc vd = views[v];
-  const uint32_t T = vd.tiles_x * vd.tiles_y;
+  const Band bd = band_of(vd, blockIdx.y);
+  if (!bd.valid) return;
+  const uint32_t rows = bd.r1 - bd.r0, tx = bd.tx, W = tx + 1;
+  for (uint32_t k = lane; k < (rows + 1) * W; k += 64) diff[k] = 0;
+  __syncthreads();
   const size_t seg = size_t(v) * n;
-  const uint32_t begin = c * chunk, end = min(n, begin + chunk);
-  uint32_t* out = matrix + matrix_off[v] + size_t(c) * T;
-  for (uint32_t band = 0; band < T; band += kBand) {
-    const uint32_t bw = min(kBand, T - band);
-    for (uint32_t t = lane; t < bw; t += 64) cnt[t] = 0;
-    __syncthreads();
-    for (uint32_t p0 = begin; p0 < end; p0 += 64) {
-      RectInfo ri = rect_info(keys, vals, rects, seg, p0 + lane, end);
-      if (__ballot(ri.cnt != 0) == 0) break;  // sorted: the rest are invalid
-      for (int j = 0; j < 64; ++j) {
-        uint32_t cj = __builtin_amdgcn_readlane(ri.cnt, j);
-        if (cj == 0) continue;
-        uint32_t x0 = __builtin_amdgcn_readlane(ri.x0, j), y0 = __builtin_amdgcn_readlane(ri.y0, j);
-        uint32_t w = __builtin_amdgcn_readlane(ri.w, j);
-        float rcp_w = 1.0f / float(w);
-        for (uint32_t t = lane; t < cj; t += 64) {
-          uint32_t tile = rect_tile(t, w, rcp_w, x0, y0, vd.tiles_x) - band;
-          if (tile < bw) atomicAdd(&cnt[tile], 1u);
-        }
-      }
-    }
-    __syncthreads();
-    for (uint32_t t = lane; t < bw; t += 64) out[band + t] = cnt[t];
-    __syncthreads();
+  const uint32_t begin = pbeg + c * chunk, end = min(pend, begin + chunk);
+  for (uint32_t p = begin + lane; p < end; p += 64) {
+    Rect r = unpack_rect(sorted[seg + p]);
+    uint32_t y0 = max(r.y0, bd.r0), y1 = min(r.y1, bd.r1 - 1);
+    if (r.x0 > r.x1 || y0 > y1) continue;
+    y0 -= bd.r0;
+    y1 -= bd.r0;
+    atomicAdd(&diff[y0 * W + r.x0], 1);
+    atomicAdd(&diff[y0 * W + r.x1 + 1], -1);
+    atomicAdd(&diff[(y1 + 1) * W + r.x0], -1);
+    atomicAdd(&diff[(y1 + 1) * W + r.x1 + 1], 1);
   }
+  __syncthreads();
+  for (uint32_t y = lane; y < rows; y += 64) {
+    int run = 0;
+    for (uint32_t x = 0; x < tx; ++x) {
+      run += diff[y * W + x];
+      diff[y * W + x] = run;
+    }
+  }
+  __syncthreads();
+  for (uint32_t x = lane; x < tx; x += 64) {
+    int run = 0;
+    for (uint32_t y = 0; y < rows; ++y) {
+      run += diff[y * W + x];
+      diff[y * W + x] = run;
+    }
+  }
+  __syncthreads();
+  const uint32_t T = tx * vd.tiles_y;
+  uint32_t* out = matrix + size_t(chunks) * vd.tile_base + size_t(c) * T + size_t(bd.r0) * tx;
+  const uint8_t* dn = done ? done + vd.tile_base + size_t(bd.r0) * tx : nullptr;
+  for (uint32_t k = lane; k < rows * tx; k += 64)
+    out[k] = (dn && dn[k]) ? 0u : uint32_t(diff[(k / tx) * W + k % tx]);
 }
 
-// tile_tot[tile_base + t] = sum over chunks of matrix[v][chunk][t].
+// tile_tot[tile_base + t] = sum over chunks of the pass's counts.
 __global__ __launch_bounds__(256) void bin_colsum_kernel(const ViewDesc* __restrict__ views, uint32_t chunks,
-                                                         const uint64_t* __restrict__ matrix_off,
                                                          const uint32_t* __restrict__ matrix,
                                                          uint32_t* __restrict__ tile_tot) {
   const uint32_t v = blockIdx.y;
@@ -280,14 +307,14 @@ __global__ __launch_bounds__(256) void bin_colsum_kernel(const ViewDesc* __restr
   const uint32_t T = vd.tiles_x * vd.tiles_y;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T) return;
-  const uint32_t* m = matrix + matrix_off[v] + t;
+  const uint32_t* m = matrix + size_t(chunks) * vd.tile_base + t;
   uint32_t s = 0;
   for (uint32_t c = 0; c < chunks; ++c) s += m[size_t(c) * T];
   tile_tot[vd.tile_base + t] = s;
 }
 
 // Exclusive scan over a view's tiles: ranges (gpu.js:468-474) and the view's
-// entry count.
+// entry count for the pass.
 __global__ __launch_bounds__(1024) void bin_tilescan_kernel(const ViewDesc* __restrict__ views, uint32_t chunks,
                                                             const uint32_t* __restrict__ tile_tot,
                                                             uint32_t* __restrict__ ranges,
@@ -311,20 +338,21 @@ __global__ __launch_bounds__(1024) void bin_tilescan_kernel(const ViewDesc* __re
   if (threadIdx.x == 0) entries[v] = carry;
 }
 
-// entry_base of each view = exclusive prefix of the entry counts.
-__global__ void bin_entry_base_kernel(ViewDesc* __restrict__ views, uint32_t num_views,
-                                      const uint64_t* __restrict__ entries) {
+// First order slot of each view's entries for the pass: a running total over
+// passes and views (*run), reset by the first pass.
+__global__ void bin_entry_base_kernel(uint32_t num_views, const uint64_t* __restrict__ entries,
+                                      uint64_t* __restrict__ ebase, uint64_t* __restrict__ run, int first) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint64_t run = 0;
+  uint64_t r = first ? 0 : *run;
   for (uint32_t v = 0; v < num_views; ++v) {
-    views[v].entry_base = run;
-    run += entries[v];
+    ebase[v] = r;
+    r += entries[v];
   }
+  *run = r;
 }
 
 // matrix[v][c][t] <- ranges start of t + sum_{c' < c} matrix[v][c'][t].
 __global__ __launch_bounds__(256) void bin_colscan_kernel(const ViewDesc* __restrict__ views, uint32_t chunks,
-                                                          const uint64_t* __restrict__ matrix_off,
                                                           const uint32_t* __restrict__ ranges,
                                                           uint32_t* __restrict__ matrix) {
   const uint32_t v = blockIdx.y;
@@ -332,7 +360,7 @@ __global__ __launch_bounds__(256) void bin_colscan_kernel(const ViewDesc* __rest
   const uint32_t T = vd.tiles_x * vd.tiles_y;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T) return;
-  uint32_t* m = matrix + matrix_off[v] + t;
+  uint32_t* m = matrix + size_t(chunks) * vd.tile_base + t;
   uint32_t run = ranges[size_t(vd.tile_base + t) * 2];
   for (uint32_t c = 0; c < chunks; ++c) {
     uint32_t x = m[size_t(c) * T];
@@ -341,52 +369,75 @@ __global__ __launch_bounds__(256) void bin_colscan_kernel(const ViewDesc* __rest
   }
 }
 
-// Pass 2: the same walk as bin_count, now handing out slots from per-tile LDS
-// cursors seeded with the scanned matrix.  Gaussians are visited in depth order
-// and a Gaussian's tiles are distinct, so each tile's slots follow (depth, index).
+// Pass 2: hands out slots from per-tile LDS cursors of the band, seeded with the
+// scanned matrix.  The wave visits its chunk's Gaussians in depth order (only
+// those whose rectangle meets the band, found by ballot); for one Gaussian the
+// lanes cover its tiles in the band, which are distinct, so every tile's slots
+// follow (depth, index).  Bands whose tiles are all done return at once.
 __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict__ views, uint32_t n, uint32_t chunk,
-                                                      const uint32_t* __restrict__ keys,
+                                                      uint32_t pbeg, uint32_t pend, uint32_t chunks,
                                                       const uint32_t* __restrict__ vals,
-                                                      const uint2* __restrict__ rects,
-                                                      const uint64_t* __restrict__ matrix_off,
+                                                      const uint2* __restrict__ sorted,
+                                                      const uint8_t* __restrict__ done,
                                                       const uint32_t* __restrict__ matrix,
+                                                      const uint64_t* __restrict__ ebase_v,
                                                       uint32_t* __restrict__ order, uint64_t capacity,
                                                       uint32_t* __restrict__ overflow) {
   extern __shared__ uint32_t cursor[];
-  const uint32_t v = blockIdx.y, c = blockIdx.x, lane = threadIdx.x;
+  const uint32_t v = blockIdx.z, c = blockIdx.x, lane = threadIdx.x;
   const ViewDesc vd = views[v];
-  const uint32_t T = vd.tiles_x * vd.tiles_y;
+  const Band bd = band_of(vd, blockIdx.y);
+  if (!bd.valid) return;
+  const uint32_t tx = bd.tx, ntile = (bd.r1 - bd.r0) * tx;
+  const uint32_t T = tx * vd.tiles_y;
+  const uint32_t* base = matrix + size_t(chunks) * vd.tile_base + size_t(c) * T + size_t(bd.r0) * tx;
+  const uint8_t* dn = done ? done + vd.tile_base + size_t(bd.r0) * tx : nullptr;
+  bool open = false;
+  for (uint32_t k = lane; k < ntile; k += 64) {
+    cursor[k] = base[k];
+    open |= !(dn && dn[k]);
+  }
+  if (__ballot(open) == 0) return;  // every tile of the band is saturated
+  // Done tiles get cursor 0xFFFFFFFF: their entries are skipped below.
+  if (dn)
+    for (uint32_t k = lane; k < ntile; k += 64)
+      if (dn[k]) cursor[k] = 0xFFFFFFFFu;
+  __syncthreads();
   const size_t seg = size_t(v) * n;
-  const uint32_t begin = c * chunk, end = min(n, begin + chunk);
-  const uint32_t* base = matrix + matrix_off[v] + size_t(c) * T;
-  const uint64_t ebase = vd.entry_base;
+  const uint32_t begin = pbeg + c * chunk, end = min(pend, begin + chunk);
+  const uint64_t ebase = ebase_v[v];
   bool over = false;
-  for (uint32_t band = 0; band < T; band += kBand) {
-    const uint32_t bw = min(kBand, T - band);
-    for (uint32_t t = lane; t < bw; t += 64) cursor[t] = base[band + t];
-    __syncthreads();
-    for (uint32_t p0 = begin; p0 < end; p0 += 64) {
-      RectInfo ri = rect_info(keys, vals, rects, seg, p0 + lane, end);
-      if (__ballot(ri.cnt != 0) == 0) break;
-      for (int j = 0; j < 64; ++j) {
-        uint32_t cj = __builtin_amdgcn_readlane(ri.cnt, j);
-        if (cj == 0) continue;
-        uint32_t x0 = __builtin_amdgcn_readlane(ri.x0, j), y0 = __builtin_amdgcn_readlane(ri.y0, j);
-        uint32_t w = __builtin_amdgcn_readlane(ri.w, j);
-        uint32_t gi = __builtin_amdgcn_readlane(ri.gi, j);
-        float rcp_w = 1.0f / float(w);
-        for (uint32_t t = lane; t < cj; t += 64) {
-          uint32_t tile = rect_tile(t, w, rcp_w, x0, y0, vd.tiles_x) - band;
-          if (tile < bw) {
-            uint32_t pos = atomicAdd(&cursor[tile], 1u);
-            uint64_t slot = ebase + pos;
-            if (slot < capacity) order[slot] = gi;
-            else over = true;
-          }
-        }
+  for (uint32_t p0 = begin; p0 < end; p0 += 64) {
+    const uint32_t p = p0 + lane;
+    Rect r{1u, 0u, 1u, 0u};
+    uint32_t gi = 0;
+    if (p < end) {
+      r = unpack_rect(sorted[seg + p]);
+      gi = vals[seg + p];
+    }
+    const uint32_t y0 = max(r.y0, bd.r0), y1 = min(r.y1, bd.r1 - 1);
+    uint64_t hits = __ballot(r.x0 <= r.x1 && y0 <= y1);
+    while (hits) {
+      const int j = __ffsll(static_cast<unsigned long long>(hits)) - 1;
+      hits &= hits - 1;
+      const uint32_t jx0 = __builtin_amdgcn_readlane(r.x0, j), jx1 = __builtin_amdgcn_readlane(r.x1, j);
+      const uint32_t jy0 = __builtin_amdgcn_readlane(y0, j) - bd.r0, jy1 = __builtin_amdgcn_readlane(y1, j) - bd.r0;
+      const uint32_t jgi = __builtin_amdgcn_readlane(gi, j);
+      const uint32_t w = jx1 - jx0 + 1, cnt = w * (jy1 - jy0 + 1);
+      const float rcp_w = 1.0f / float(w);
+      for (uint32_t t = lane; t < cnt; t += 64) {
+        uint32_t row = uint32_t(float(t) * rcp_w);
+        while (row * w > t) --row;
+        while ((row + 1) * w <= t) ++row;
+        const uint32_t idx = (jy0 + row) * tx + jx0 + (t - row * w);
+        const uint32_t pos = cursor[idx];
+        if (pos == 0xFFFFFFFFu) continue;  // done tile
+        cursor[idx] = pos + 1;
+        const uint64_t slot = ebase + pos;
+        if (slot < capacity) order[slot] = jgi;
+        else over = true;
       }
     }
-    __syncthreads();
   }
   if (over) atomicOr(overflow, 1u);
 }
@@ -425,44 +476,71 @@ void launch_depth_sort(hipStream_t s, uint32_t* keys, uint32_t n, uint32_t num_v
 }
 
 size_t bin_matrix_elems(const ViewDesc* host_views, uint32_t num_views, uint32_t n, uint32_t chunk) {
-  size_t chunks = (size_t(n) + chunk - 1) / chunk;
+  size_t chunks = (size_t(n) + chunk - 1) / chunk + 1;  // +1: a slice may start mid-chunk
   size_t total = 0;
   for (uint32_t v = 0; v < num_views; ++v) total += chunks * host_views[v].tiles_x * host_views[v].tiles_y;
   return total;
 }
 
-void launch_bin_count(hipStream_t s, const ViewDesc* views, uint32_t num_views, uint32_t n, uint32_t chunk,
-                      const uint32_t* keys, const uint32_t* vals, const uint2* rects, const BinScratch& bs,
-                      uint32_t* ranges, uint32_t max_tiles_per_view) {
-  if (num_views == 0) return;
-  const uint32_t chunks = n == 0 ? 0 : (n + chunk - 1) / chunk;
-  const uint32_t band = max_tiles_per_view < kBand ? max_tiles_per_view : kBand;
-  const size_t lds = size_t(band) * sizeof(uint32_t);
-  if (chunks > 0) {
-    hipLaunchKernelGGL(bin_count_kernel, dim3(chunks, num_views), dim3(64), lds, s, views, n, chunk, chunks, keys,
-                       vals, rects, bs.matrix_off, bs.matrix);
-    dim3 tgrid((max_tiles_per_view + 255) / 256, num_views);
-    hipLaunchKernelGGL(bin_colsum_kernel, tgrid, dim3(256), 0, s, views, chunks, bs.matrix_off, bs.matrix,
-                       bs.tile_tot);
+namespace {
+struct BandGeometry {
+  uint32_t max_bands = 1, max_tiles = 1;
+  size_t count_lds = 4, emit_lds = 4;
+};
+
+BandGeometry band_geometry(const ViewDesc* host_views, uint32_t num_views) {
+  BandGeometry g;
+  for (uint32_t v = 0; v < num_views; ++v) {
+    const uint32_t tx = host_views[v].tiles_x, ty = host_views[v].tiles_y;
+    const uint32_t R = band_rows(tx, ty);
+    g.max_bands = std::max(g.max_bands, (ty + R - 1) / R);
+    g.max_tiles = std::max(g.max_tiles, tx * ty);
+    g.count_lds = std::max(g.count_lds, size_t(R + 1) * (tx + 1) * sizeof(int));
+    g.emit_lds = std::max(g.emit_lds, size_t(R) * tx * sizeof(uint32_t));
   }
-  hipLaunchKernelGGL(bin_tilescan_kernel, dim3(num_views), dim3(1024), 0, s, views, chunks, bs.tile_tot, ranges,
-                     bs.entries);
-  hipLaunchKernelGGL(bin_entry_base_kernel, dim3(1), dim3(64), 0, s, const_cast<ViewDesc*>(views), num_views,
-                     bs.entries);
+  return g;
 }
 
-void launch_bin_emit(hipStream_t s, const ViewDesc* views, uint32_t num_views, uint32_t n, uint32_t chunk,
-                     const uint32_t* keys, const uint32_t* vals, const uint2* rects, const BinScratch& bs,
-                     const uint32_t* ranges, uint32_t* order, uint64_t order_capacity, uint32_t* overflow,
-                     uint32_t max_tiles_per_view) {
-  const uint32_t chunks = n == 0 ? 0 : (n + chunk - 1) / chunk;
+uint32_t slice_chunks(const BinPass& p, uint32_t chunk) {
+  return p.pend > p.pbeg ? (p.pend - p.pbeg + chunk - 1) / chunk : 0;
+}
+}  // namespace
+
+void launch_bin_gather(hipStream_t s, uint32_t num_views, uint32_t n, const uint32_t* vals, const uint2* rects,
+                       uint2* rect_sorted) {
+  if (n == 0 || num_views == 0) return;
+  hipLaunchKernelGGL(bin_gather_kernel, dim3((n + 255) / 256, num_views), dim3(256), 0, s, vals, rects, n,
+                     rect_sorted);
+}
+
+void launch_bin_count(hipStream_t s, const ViewDesc* views, const ViewDesc* host_views, uint32_t num_views,
+                      uint32_t n, uint32_t chunk, const BinScratch& bs, const BinPass& pass) {
+  if (num_views == 0) return;
+  const uint32_t chunks = slice_chunks(pass, chunk);
+  const BandGeometry g = band_geometry(host_views, num_views);
+  if (chunks > 0) {
+    hipLaunchKernelGGL(bin_count_kernel, dim3(chunks, g.max_bands, num_views), dim3(64), g.count_lds, s, views, n,
+                       chunk, pass.pbeg, pass.pend, chunks, bs.rect_sorted, pass.done, bs.matrix);
+    dim3 tgrid((g.max_tiles + 255) / 256, num_views);
+    hipLaunchKernelGGL(bin_colsum_kernel, tgrid, dim3(256), 0, s, views, chunks, bs.matrix, bs.tile_tot);
+  }
+  hipLaunchKernelGGL(bin_tilescan_kernel, dim3(num_views), dim3(1024), 0, s, views, chunks, bs.tile_tot, pass.ranges,
+                     pass.entries);
+  hipLaunchKernelGGL(bin_entry_base_kernel, dim3(1), dim3(64), 0, s, num_views, pass.entries, pass.ebase, bs.run,
+                     pass.first ? 1 : 0);
+}
+
+void launch_bin_emit(hipStream_t s, const ViewDesc* views, const ViewDesc* host_views, uint32_t num_views,
+                     uint32_t n, uint32_t chunk, const uint32_t* vals, const BinScratch& bs, const BinPass& pass,
+                     uint32_t* order, uint64_t order_capacity, uint32_t* overflow) {
+  const uint32_t chunks = slice_chunks(pass, chunk);
   if (num_views == 0 || chunks == 0) return;
-  const uint32_t band = max_tiles_per_view < kBand ? max_tiles_per_view : kBand;
-  const size_t lds = size_t(band) * sizeof(uint32_t);
-  dim3 tgrid((max_tiles_per_view + 255) / 256, num_views);
-  hipLaunchKernelGGL(bin_colscan_kernel, tgrid, dim3(256), 0, s, views, chunks, bs.matrix_off, ranges, bs.matrix);
-  hipLaunchKernelGGL(bin_emit_kernel, dim3(chunks, num_views), dim3(64), lds, s, views, n, chunk, keys, vals, rects,
-                     bs.matrix_off, bs.matrix, order, order_capacity, overflow);
+  const BandGeometry g = band_geometry(host_views, num_views);
+  dim3 tgrid((g.max_tiles + 255) / 256, num_views);
+  hipLaunchKernelGGL(bin_colscan_kernel, tgrid, dim3(256), 0, s, views, chunks, pass.ranges, bs.matrix);
+  hipLaunchKernelGGL(bin_emit_kernel, dim3(chunks, g.max_bands, num_views), dim3(64), g.emit_lds, s, views, n, chunk,
+                     pass.pbeg, pass.pend, chunks, vals, bs.rect_sorted, pass.done, bs.matrix, pass.ebase, order,
+                     order_capacity, overflow);
 }
 
 }  // namespace gsr

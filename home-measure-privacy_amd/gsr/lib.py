@@ -24,6 +24,7 @@ EXPORTS = (
     "gsr_clear", "gsr_synchronize", "gsr_project", "gsr_bin_sort", "gsr_forward", "gsr_loss_grad", "gsr_backward",
     "gsr_shape", "gsr_adam", "gsr_stats", "gsr_batch_create", "gsr_batch_destroy", "gsr_batch_set_cameras",
     "gsr_batch_step", "gsr_batch_entries", "gsr_batch_configure", "gsr_batch_autotune", "gsr_batch_profile",
+    "gsr_batch_set_passes",
 )
 STAGES = ("project", "depth_sort", "bin_count", "bin_emit", "forward", "loss_grad", "backward", "chain")
 
@@ -86,6 +87,7 @@ def load():
         "gsr_batch_configure": (i32, [P, u32, u32]),
         "gsr_batch_autotune": (i32, [P, P, P, P, f32, P, P, P, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "gsr_batch_profile": (i32, [P, P, P, P, f32, P, P, P, P, P]),
+        "gsr_batch_set_passes": (i32, [P, u32, u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -254,17 +256,21 @@ class Batch:
         self.ctx._check(self.lib.gsr_batch_entries(self.h, out))
         return np.array(out[:], dtype=np.uint64)
 
+    def set_passes(self, passes: int, growth: int = 2):
+        self.ctx._check(self.lib.gsr_batch_set_passes(self.h, passes, growth))
+
     def configure(self, tile_batch: int = 0, chunk: int = 0):
         self.ctx._check(self.lib.gsr_batch_configure(self.h, tile_batch, chunk))
 
     def profile(self, gs, sh, targets, lam, grads, shape, sh_grad=None):
         """One eager step timed per stage with HIP events: (ms per stage dict,
-        counters [V, 3] of entries E, saturated-prefix entries F, touched G)."""
+        counters [V, 4] of entries E, saturated-prefix entries F, touched
+        Gaussians G, walked entries W)."""
         ms = (ctypes.c_float * len(STAGES))()
-        cnt = (ctypes.c_uint64 * (3 * self.V))()
+        cnt = (ctypes.c_uint64 * (4 * self.V))()
         self.ctx._check(self.lib.gsr_batch_profile(self.h, gs.p, sh.p if sh else None, targets.p, lam, grads.p,
                                                    shape.p, sh_grad.p if sh_grad else None, ms, cnt))
-        return dict(zip(STAGES, [float(x) for x in ms])), np.array(cnt[:], dtype=np.uint64).reshape(self.V, 3)
+        return dict(zip(STAGES, [float(x) for x in ms])), np.array(cnt[:], dtype=np.uint64).reshape(self.V, 4)
 
     def autotune(self, gs, sh, targets, lam, grads, shape, sh_grad=None):
         tb, ch = ctypes.c_uint32(0), ctypes.c_uint32(0)

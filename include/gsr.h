@@ -165,12 +165,21 @@ int gsr_batch_entries(gsr_batch* batch, uint64_t* entries);
  * binning chunk (0 = keep).  gsr_batch_autotune times candidates on the given
  * inputs and keeps the fastest; it writes the choice to *tile_batch / *chunk. */
 int gsr_batch_configure(gsr_batch* batch, uint32_t tile_batch, uint32_t chunk);
+/* Depth-slice passes of the batched step (1 to 8; default 2 with growth 8 for
+ * n >= 65536, else 1).  With k > 1 passes the depth-sorted Gaussians are binned
+ * in slices whose ends grow by `growth` (2 to 16; the first slice holds
+ * ceil(n / growth^(k-1))), and a slice is binned only for tiles whose pixels
+ * the slices before it left unsaturated (T > 1e-4).  Images and gradients are
+ * those of the full tile lists: a saturated tile's blend stops before any later
+ * entry.  growth is ignored for passes == 1. */
+int gsr_batch_set_passes(gsr_batch* batch, uint32_t passes, uint32_t growth);
 /* Runs one step without the graph, timing each stage with HIP events on the
  * context's stream.  stage_ms[GSR_STAGES] receives milliseconds for: project,
  * depth sort, bin count, bin emit, forward, loss gradient, backward, chain.
- * counters[3 * num_views] receives per view: entries E, saturated-prefix entries
- * F (sum over tiles of the furthest pixel's last hit minus the tile start) and
- * touched Gaussians G (any gradient in that view). */
+ * counters[4 * num_views] receives per view: entries E, saturated-prefix entries
+ * F (sum over tiles of the furthest pixel's last hit minus the tile start),
+ * touched Gaussians G (any gradient in that view) and walked entries W (entries
+ * the forward examined before its tiles saturated or ended). */
 #define GSR_STAGES 8
 int gsr_batch_profile(gsr_batch* batch, const float* gs, const float* sh, const float* targets, float lambda,
                       float* grads, float* shape, float* sh_grad, float* stage_ms, uint64_t* counters);
