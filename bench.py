@@ -132,6 +132,26 @@ def cpu_baseline(sc, budget_s: float):
                       f"{dt:.1f} s"}
 
 
+def copy_calibration(mib: int):
+    """Device copy of known size (the library's copy kernel), for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run to bytes."""
+    ctx = lib.Context(0)
+    nbytes = mib << 20
+    a, b = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    ctx.copy(b, a, nbytes)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        ctx.copy(b, a, nbytes)
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / 5
+    print(json.dumps({"copy_bytes": nbytes, "read_bytes": nbytes, "write_bytes": nbytes,
+                      "GBps_read_plus_write": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
+    a.free()
+    b.free()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,11 +166,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0, help="only time a device copy of this many MiB")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design shards views and "
               f"all-reduces gradients but is not built in this tree.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_mib:
+        copy_calibration(args.copy_mib)
+        return
 
     n, V = args.gaussians, args.views
     log(f"scene: {n} Gaussians, {V} views")

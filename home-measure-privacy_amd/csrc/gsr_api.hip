@@ -214,8 +214,20 @@ int gsr_download(gsr_ctx* ctx, void* dst, const void* src, size_t bytes) {
 
 int gsr_clear(gsr_ctx* ctx, void* dptr, size_t bytes) {
   if (!ctx || (!dptr && bytes)) return fail(ctx, GSR_E_INVALID, "NULL argument");
-  if (bytes % 4) return fail(ctx, GSR_E_INVALID, "clear size %zu is not a multiple of 4", bytes);
+  if (bytes % 4 || reinterpret_cast<uintptr_t>(dptr) % 4)
+    return fail(ctx, GSR_E_INVALID, "clear of %zu bytes at %p is not 4-byte aligned", bytes, dptr);
   launch_zero(ctx->stream, dptr, bytes);
+  GSR_LAUNCHED(ctx);
+  return GSR_OK;
+}
+
+int gsr_copy(gsr_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || ((!dst || !src) && bytes)) return fail(ctx, GSR_E_INVALID, "NULL argument");
+  if (bytes % 4 || reinterpret_cast<uintptr_t>(dst) % 4 || reinterpret_cast<uintptr_t>(src) % 4)
+    return fail(ctx, GSR_E_INVALID, "copy of %zu bytes is not 4-byte aligned", bytes);
+  const uintptr_t d = reinterpret_cast<uintptr_t>(dst), s = reinterpret_cast<uintptr_t>(src);
+  if (bytes && d < s + bytes && s < d + bytes) return fail(ctx, GSR_E_INVALID, "copy regions overlap");
+  launch_copy(ctx->stream, dst, src, bytes);
   GSR_LAUNCHED(ctx);
   return GSR_OK;
 }

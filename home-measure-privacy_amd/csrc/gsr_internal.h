@@ -131,6 +131,8 @@ void launch_backward(hipStream_t s, const RasterArgs& a, int persistent, uint32_
 void raster_occupancy(int* forward_per_cu, int* backward_per_cu);
 void launch_loss_grad(hipStream_t s, const ViewDesc* views, const ViewDesc* host_views, uint32_t num_views,
                       const float* image, const float* target, float lambda, float* coef, float* grad);
+// Device copy / zero fill of 4-byte aligned memory, bytes a multiple of 4.
+void launch_copy(hipStream_t s, void* dst, const void* src, size_t bytes);
 void launch_zero(hipStream_t s, void* p, size_t bytes);
 
 // Chain rule to the 3D parameters, Adam, stats (gsr_chain.hip).

@@ -6,14 +6,18 @@
 // LDS (each lane gathers one 48-byte splat record; the next batch is prefetched
 // into registers while the current one is blended), every lane then reads the
 // batch back as broadcasts.  A wave stops as soon as all of its 256 pixels have
-// T <= 1e-4, so saturated tiles retire early.
+// T <= 1e-4, so saturated tiles retire early.  Staging also computes each
+// entry's hit box (where alpha can reach 1/255); entries whose box misses the
+// tile's active pixels, and 4-row bands it misses, are skipped without changing
+// any result.
 //
 // The backward walks the same list front to back and recomputes transmittance
 // with exactly the forward's arithmetic, so it stops at the same entry for each
 // pixel.  The colour behind an entry is C_final - (accumulated colour), C_final
 // being the forward's image.  Per entry, the four pixels of a lane are summed in
-// registers, the wave reduces with DPP, and nine lanes issue one float atomic
-// each into the 16-float gradient record.
+// registers, the nine terms are reduced over the wave together (cross-half lane
+// swaps, then DPP within rows), and nine lanes issue one float atomic each into
+// the 16-float gradient record.
 #include "gsr_internal.h"
 
 namespace gsr {
@@ -85,8 +89,44 @@ __device__ inline Batch fetch_batch(const Tile& t, uint32_t k0, uint32_t lane) {
   return b;
 }
 
+constexpr float kNever = -1e30f;  // extent of an entry that cannot hit any pixel
+
+// Half-extents (ex, ey) of the pixel offsets (dx, dy) at which alphaAt
+// (gpu.js:92-103) can return a hit: inside its square |dx|, |dy| <= r and inside
+// the ellipse where opacity * exp(power) >= 1/255, i.e. -2 power <= 2 ln(255 op)
+// with -2 power = [dx dy] [[A B] [B C]] [dx dy]^T, whose x half-width is
+// sqrt(2 t C / det) and y half-width sqrt(2 t A / det).  The threshold t is raised
+// by 0.01 plus a bound on the float32 rounding of the exponent's terms, det is
+// Kahan's compensated A C - B^2, and the widths are padded, so every pixel outside
+// the box evaluates to a miss in the blend's own arithmetic too and skipping it
+// changes no result.  Invalid splats and opacities below ~1/255 get kNever;
+// degenerate conics fall back to the square.
+__device__ inline float2 hit_extent(const Batch& b) {
+  const float A = b.r0.z, B = b.r0.w, C = b.r1.x, op = b.r1.y, r = b.r1.z;
+  if (b.r2.w < 0.5f) return make_float2(kNever, kNever);
+  const float r1 = r + 1.0f;
+  const float slack = 0.01f + 1e-6f * (fabsf(A) + 2.0f * fabsf(B) + fabsf(C)) * r1 * r1;
+  const float t = __logf(255.0f * op) + slack;
+  const float bb = B * B;
+  const float det = fmaf(A, C, -bb) - fmaf(B, B, -bb);
+  if (t != t || !(det > 0.0f) || !(A > 0.0f) || !(C > 0.0f)) return make_float2(r, r);
+  if (!(t > 0.0f)) return make_float2(kNever, kNever);
+  const float k = 2.0f * t / det;
+  const float ex = fmaf(sqrtf(k * C), 1.0001f, 1e-3f);
+  const float ey = fmaf(sqrtf(k * A), 1.0001f, 1e-3f);
+  return make_float2(fminf(r, ex), fminf(r, ey));
+}
+
+// LDS layout of a staged entry: (u, v, A, B), (C, opacity, radius, ex), (r, g, b, ey).
+__device__ inline void stage(float4 (*lds)[3], uint32_t lane, const Batch& b) {
+  const float2 e = hit_extent(b);
+  lds[lane][0] = b.r0;
+  lds[lane][1] = make_float4(b.r1.x, b.r1.y, b.r1.z, e.x);
+  lds[lane][2] = make_float4(b.r2.x, b.r2.y, b.r2.z, e.y);
+}
+
 // Per-entry conic coefficients of the exponent in base 2:
-// power * log2(e) = (qA dx^2 + qC dy^2) - qB dx dy with qA = -A/2 log2 e,
+// power * log2(e) = qA dx^2 + dy (qC dy - qB dx) with qA = -A/2 log2 e,
 // qB = B log2 e, qC = -C/2 log2 e.  The sign (the power > 0 test) is unchanged.
 struct Conic2 {
   float qA, qB, qC;
@@ -97,12 +137,25 @@ __device__ inline Conic2 conic2(float A, float B, float C) {
   return Conic2{-0.5f * l2e * A, l2e * B, -0.5f * l2e * C};
 }
 
-// alphaAt (gpu.js:92-103) without branches: alpha, or 0 for a miss.  The miss
-// tests are the reference's, negated as written there (a NaN takes the same path).
-__device__ inline float alpha_at(float dx, float dy, const Conic2& q, float op, float rad) {
-  const float p2 = (q.qA * dx * dx + q.qC * dy * dy) - q.qB * dx * dy;
+// Terms of the exponent shared by the pixels of one lane column.
+struct Row {
+  float qadx2;  // qA dx^2
+  float nqbdx;  // -qB dx
+  bool missx;   // |dx| > r
+};
+
+__device__ inline Row row_terms(const Conic2& q, float dx, float rad) {
+  return Row{q.qA * dx * dx, -(q.qB * dx), fabsf(dx) > rad};
+}
+
+// alphaAt (gpu.js:92-103) without branches: alpha, or 0 for a miss or for a pixel
+// already done (T <= 1e-4).  The miss tests are the reference's, negated as
+// written there (a NaN takes the same path).  The forward and the backward call
+// this same function, so both see the same alpha bit for bit.
+__device__ inline float alpha_at(float dy, const Conic2& q, const Row& w, float op, float rad, float T) {
+  const float p2 = fmaf(dy, fmaf(q.qC, dy, w.nqbdx), w.qadx2);
   const float alpha = fminf(kAlphaMax, op * __builtin_amdgcn_exp2f(p2));
-  const bool miss = fabsf(dx) > rad || fabsf(dy) > rad || p2 > 0.0f || alpha < kAlphaMin;
+  const bool miss = w.missx || fabsf(dy) > rad || p2 > 0.0f || alpha < kAlphaMin || !(T > kTStop);
   return miss ? 0.0f : alpha;
 }
 
@@ -138,11 +191,17 @@ __device__ inline Box active_box(float px, const float py[kPix], const float T[k
   return b;
 }
 
-// True when the splat's square footprint (|dx| <= r and |dy| <= r, alphaAt's
-// cull) misses every active pixel.  Uses alphaAt's own subtraction px - u, which
-// is monotonic in px, so the test is exact at the box edges.
-__device__ inline bool box_culled(const Box& b, float u, float v, float r) {
-  return (b.x1 - u) < -r || (b.x0 - u) > r || (b.y1 - v) < -r || (b.y0 - v) > r;
+// True when the entry's hit box (|dx| <= ex, |dy| <= ey) misses every active
+// pixel.  Uses alphaAt's own subtraction px - u, which is monotonic in px, so the
+// test agrees with the per-pixel values at the box edges.
+__device__ inline bool box_culled(const Box& b, float u, float v, float ex, float ey) {
+  return (b.x1 - u) < -ex || (b.x0 - u) > ex || (b.y1 - v) < -ey || (b.y0 - v) > ey;
+}
+
+// Lane rows (lane >> 4) + 4 i form band i: tile rows 4 i .. 4 i + 3.  True when
+// the entry's hit box misses the band's pixel centres [ylo, ylo + 3].
+__device__ inline bool band_culled(float ylo, float v, float ey) {
+  return (ylo + 3.0f - v) < -ey || (ylo - v) > ey;
 }
 
 // ---- forward -------------------------------------------------------------------
@@ -176,14 +235,13 @@ __device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) 
       cb[i] = a.image[pix * 3 + 2];
     }
   }
+  const float band0 = float(t.y0) + 0.5f;
   Batch nb = fetch_batch(t, t.start, lane);
   uint32_t walked = t.start;
   for (uint32_t k0 = t.start; k0 < t.end; k0 += 64) {
     if (!any_active(T)) break;
     __syncthreads();
-    lds[lane][0] = nb.r0;
-    lds[lane][1] = nb.r1;
-    lds[lane][2] = nb.r2;
+    stage(lds, lane, nb);
     __syncthreads();
     if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
     const uint32_t nk = min(64u, t.end - k0);
@@ -194,16 +252,17 @@ __device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) 
         if ((j & 31) == 0) box = active_box(px, py, T);  // a stale box is a superset: still exact
       }
       walked = k0 + j + 1;
-      const float4 r0 = lds[j][0], r1 = lds[j][1];
-      if (box_culled(box, r0.x, r0.y, r1.z)) continue;
-      const float4 r2 = lds[j][2];
-      if (r2.w < 0.5f) continue;
+      const float4 r0 = lds[j][0], r1 = lds[j][1], r2 = lds[j][2];
+      const float u = r0.x, v = r0.y, op = r1.y, rad = r1.z, ex = r1.w, ey = r2.w;
+      if (box_culled(box, u, v, ex, ey)) continue;
       const Conic2 q = conic2(r0.z, r0.w, r1.x);
+      const Row row = row_terms(q, px - u, rad);
 #pragma unroll
       for (int i = 0; i < kPix; ++i) {
-        // A miss (or a done pixel) blends with weight 0: colour and T unchanged.
-        float alpha = alpha_at(px - r0.x, py[i] - r0.y, q, r1.y, r1.z);
-        alpha = T[i] > kTStop ? alpha : 0.0f;
+        // A band outside the hit box, a miss or a done pixel blends with weight 0:
+        // colour and T unchanged.
+        if (band_culled(band0 + 4.0f * i, v, ey)) continue;
+        const float alpha = alpha_at(py[i] - v, q, row, op, rad, T[i]);
         const float wgt = alpha * T[i];
         cr[i] = fmaf(r2.x, wgt, cr[i]);
         cg[i] = fmaf(r2.y, wgt, cg[i]);
@@ -275,49 +334,82 @@ __global__ __launch_bounds__(64) void forward_persistent_kernel(RasterArgs a) {
 
 // ---- backward ------------------------------------------------------------------
 
-template <int CTRL, int ROW_MASK = 0xF>
+template <int CTRL>
 __device__ inline float dpp_src(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
-// Sum over the wave; the total lands in lane 63.
-__device__ inline float wave_sum63(float v) {
-  v += dpp_src<0xB1>(v);        // quad_perm [1,0,3,2]
-  v += dpp_src<0x4E>(v);        // quad_perm [2,3,0,1]
-  v += dpp_src<0x124>(v);       // row_ror:4
-  v += dpp_src<0x128>(v);       // row_ror:8  -> every lane holds its row's sum
-  v += dpp_src<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
-  v += dpp_src<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+// Sum over each row of 16 lanes, left in every lane of the row.
+__device__ inline float row_sum(float v) {
+  v += dpp_src<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_src<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_src<0x124>(v);  // row_ror:4
+  v += dpp_src<0x128>(v);  // row_ror:8
   return v;
 }
 
-__device__ inline float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+// Lanes l and l + 32 added: lanes 0-31 receive a's sums, lanes 32-63 b's.
+__device__ inline float fold32(float a, float b) {
+  const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  return __uint_as_float(s[0]) + __uint_as_float(s[1]);
+}
+
+// Rows 0+1 and 2+3 added: rows 0 and 2 receive a's sums, rows 1 and 3 b's.
+__device__ inline float fold16(float a, float b) {
+  const auto s = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  return __uint_as_float(s[0]) + __uint_as_float(s[1]);
+}
+
+// Wave sums of the nine gradient terms by transposition: two folds halve the
+// lanes while pairing values, then rows of 16 are summed.  Lane (16 row + col)
+// ends with the total of term reduce9_slot(lane) (col 0: terms 0 2 1 3 by row,
+// col 1: 4 6 5 7, lane 2: term 8); the result is that lane's value.
+__device__ inline float reduce9(const float g[9], uint32_t lane) {
+  const float h0 = fold32(g[0], g[1]), h1 = fold32(g[2], g[3]);
+  const float h2 = fold32(g[4], g[5]), h3 = fold32(g[6], g[7]);
+  const float h4 = fold32(g[8], 0.0f);
+  const float k0 = row_sum(fold16(h0, h1));
+  const float k1 = row_sum(fold16(h2, h3));
+  const float k2 = row_sum(fold16(h4, 0.0f));
+  const uint32_t col = lane & 15;
+  return col == 0 ? k0 : (col == 1 ? k1 : k2);
+}
+
+// Gradient slot summed into lane `lane` by reduce9, or -1 for none.
+__device__ inline int reduce9_slot(uint32_t lane) {
+  const uint32_t row = lane >> 4, col = lane & 15;
+  const int by_row = int((row & 1) * 2 + (row >> 1));  // rows 0 1 2 3 -> 0 2 1 3
+  if (col == 0) return by_row;
+  if (col == 1) return 4 + by_row;
+  return lane == 2 ? 8 : -1;
+}
 
 __device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3], uint32_t* gi_lds) {
   const uint32_t lane = __lane_id();
   Tile t = load_tile(a, w);
   const uint32_t x = t.x0 + (lane & 15);
   const float px = float(x) + 0.5f;
-  float py[kPix], T[kPix], ar[kPix], ag[kPix], ab[kPix];
-  float fr[kPix], fg[kPix], fb[kPix], dr[kPix], dg[kPix], db[kPix];
+  const int slot = reduce9_slot(lane);
+  // Per pixel: dL/dC (d), sum_c dL/dC_c * C_final_c (DF) and the running
+  // sum_c dL/dC_c * accumulated_c (S); the colour behind an entry, weighted by
+  // dL/dC, is DF - S (gpu.js:266-268 with behind = C_final - accumulated).
+  float py[kPix], T[kPix], S[kPix], DF[kPix], dr[kPix], dg[kPix], db[kPix];
 #pragma unroll
   for (int i = 0; i < kPix; ++i) {
     uint32_t y = t.y0 + (lane >> 4) + 4 * i;
     py[i] = float(y) + 0.5f;
     bool inside = x < t.width && y < t.height;
     T[i] = inside ? 1.0f : 0.0f;
-    ar[i] = ag[i] = ab[i] = 0.0f;
-    fr[i] = fg[i] = fb[i] = dr[i] = dg[i] = db[i] = 0.0f;
+    S[i] = DF[i] = dr[i] = dg[i] = db[i] = 0.0f;
     if (inside) {
       size_t pix = t.pixel_base + size_t(y) * t.width + x;
-      fr[i] = a.image[pix * 3];
-      fg[i] = a.image[pix * 3 + 1];
-      fb[i] = a.image[pix * 3 + 2];
       dr[i] = a.dl[pix * 3];
       dg[i] = a.dl[pix * 3 + 1];
       db[i] = a.dl[pix * 3 + 2];
+      DF[i] = fmaf(dr[i], a.image[pix * 3], fmaf(dg[i], a.image[pix * 3 + 1], db[i] * a.image[pix * 3 + 2]));
     }
   }
+  const float band0 = float(t.y0) + 0.5f;
   float* grads = a.grads + size_t(t.view) * a.grads_view_stride;
   // The passes of a tile follow each other in depth order; pixel state carries over.
   for (int p = 0; p < a.num_passes; ++p) {
@@ -326,9 +418,7 @@ __device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3],
   for (uint32_t k0 = t.start; k0 < t.end; k0 += 64) {
     if (!any_active(T)) break;
     __syncthreads();
-    lds[lane][0] = nb.r0;
-    lds[lane][1] = nb.r1;
-    lds[lane][2] = nb.r2;
+    stage(lds, lane, nb);
     gi_lds[lane] = nb.gi;
     __syncthreads();
     if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
@@ -339,62 +429,53 @@ __device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3],
         if (!any_active(T)) break;
         box = active_box(px, py, T);
       }
-      const float4 r0 = lds[j][0], r1 = lds[j][1];
-      if (box_culled(box, r0.x, r0.y, r1.z)) continue;
-      const float4 r2 = lds[j][2];
-      if (r2.w < 0.5f) continue;
-      const float A = r0.z, B = r0.w, C = r1.x, op = r1.y;
+      const float4 r0 = lds[j][0], r1 = lds[j][1], r2 = lds[j][2];
+      const float u = r0.x, v = r0.y, A = r0.z, B = r0.w, C = r1.x, op = r1.y, rad = r1.z;
+      if (box_culled(box, u, v, r1.w, r2.w)) continue;
       const Conic2 q = conic2(A, B, C);
+      const float dx = px - u;
+      const Row row = row_terms(q, dx, rad);
       const float omo = 1.0f - op;
+      const float Adx = A * dx, Bdx = B * dx, ndx = -dx, nhdx2 = -0.5f * dx * dx;
       float g[9];
 #pragma unroll
       for (int m = 0; m < 9; ++m) g[m] = 0.0f;
       bool hit = false;
 #pragma unroll
       for (int i = 0; i < kPix; ++i) {
-        const float dx = px - r0.x, dy = py[i] - r0.y;
-        float alpha = alpha_at(dx, dy, q, op, r1.z);
-        alpha = T[i] > kTStop ? alpha : 0.0f;
+        if (band_culled(band0 + 4.0f * i, v, r2.w)) continue;
+        const float dy = py[i] - v;
+        // Same alpha and transmittance as forward_tile, bit for bit.
+        const float alpha = alpha_at(dy, q, row, op, rad, T[i]);
         const bool h = alpha != 0.0f;
         hit |= h;
-        // Same arithmetic as forward_tile: acc and T follow it bit for bit (a
-        // miss has weight 0 and leaves both unchanged).
-        const float wgt = alpha * T[i];
-        ar[i] = fmaf(r2.x, wgt, ar[i]);
-        ag[i] = fmaf(r2.y, wgt, ag[i]);
-        ab[i] = fmaf(r2.z, wgt, ab[i]);
         const float Tk = T[i];
+        const float wgt = alpha * Tk;
         const float oma = 1.0f - alpha;
         T[i] = Tk * oma;
-        // gpu.js:266-268: colour gradient and dLoss/dalpha; behind = C_final - acc.
-        g[0] += dr[i] * wgt;
-        g[1] += dg[i] * wgt;
-        g[2] += db[i] * wgt;
+        // gpu.js:266-268: colour gradient and dLoss/dalpha.
+        g[0] = fmaf(dr[i], wgt, g[0]);
+        g[1] = fmaf(dg[i], wgt, g[1]);
+        g[2] = fmaf(db[i], wgt, g[2]);
+        const float dc = fmaf(dr[i], r2.x, fmaf(dg[i], r2.y, db[i] * r2.z));
+        S[i] = fmaf(wgt, dc, S[i]);
         const float inv = __builtin_amdgcn_rcpf(oma);
-        float dA = dr[i] * (Tk * r2.x - (fr[i] - ar[i]) * inv) + dg[i] * (Tk * r2.y - (fg[i] - ag[i]) * inv) +
-                   db[i] * (Tk * r2.z - (fb[i] - ab[i]) * inv);
+        float dA = fmaf(Tk, dc, -((DF[i] - S[i]) * inv));
         // gpu.js:270-281: opacity logit and screen (u, v, conic) gradients, only
         // while the 0.99 clamp is not active.
         dA = (h && alpha < kAlphaMax) ? dA : 0.0f;
-        g[3] += dA * alpha * omo;
         const float dP = dA * alpha;
-        g[4] += dP * (A * dx + B * dy);
-        g[5] += dP * (C * dy + B * dx);
-        g[6] += dP * -0.5f * dx * dx;
-        g[7] += dP * -dx * dy;
-        g[8] += dP * -0.5f * dy * dy;
+        const float dPdy = dP * dy;
+        g[3] = fmaf(dP, omo, g[3]);
+        g[4] = fmaf(dPdy, B, fmaf(dP, Adx, g[4]));
+        g[5] = fmaf(dPdy, C, fmaf(dP, Bdx, g[5]));
+        g[6] = fmaf(dP, nhdx2, g[6]);
+        g[7] = fmaf(dPdy, ndx, g[7]);
+        g[8] = fmaf(dPdy, -0.5f * dy, g[8]);
       }
       if (__ballot(hit) == 0) continue;
-      float s[9];
-#pragma unroll
-      for (int m = 0; m < 9; ++m) s[m] = lane63(wave_sum63(g[m]));
-      float val = 0.0f;
-#pragma unroll
-      for (int m = 0; m < 9; ++m) val = lane == uint32_t(m) ? s[m] : val;
-      if (lane < 9 && val != 0.0f) {
-        const uint32_t gi = gi_lds[j];
-        unsafeAtomicAdd(grads + size_t(gi) * 16 + lane, val);
-      }
+      const float val = reduce9(g, lane);
+      if (slot >= 0 && val != 0.0f) unsafeAtomicAdd(grads + size_t(gi_lds[j]) * 16 + slot, val);
     }
   }
   }
@@ -494,12 +575,26 @@ __global__ __launch_bounds__(256) void loss_grad_kernel(const ViewDesc* __restri
   }
 }
 
-__global__ __launch_bounds__(256) void zero_kernel(uint4* __restrict__ p, size_t n16, uint32_t* __restrict__ tail,
-                                                   uint32_t ntail) {
-  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+// Copies src to dst (or zeroes dst when src is null), `words` 32-bit words.  The
+// words [head, head + 4 * n16) move as 16-byte vectors (dst and src share their
+// 16-byte phase there); the words outside that body move one by one.
+__global__ __launch_bounds__(256) void copy_words_kernel(uint32_t* __restrict__ dst,
+                                                         const uint32_t* __restrict__ src, size_t words,
+                                                         size_t head, size_t n16) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const size_t stride = size_t(gridDim.x) * blockDim.x;
-  for (; i < n16; i += stride) p[i] = make_uint4(0u, 0u, 0u, 0u);
-  if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0u;
+  uint4* d4 = reinterpret_cast<uint4*>(dst + head);
+  if (src) {
+    const uint4* s4 = reinterpret_cast<const uint4*>(src + head);
+    for (size_t k = i; k < n16; k += stride) d4[k] = s4[k];
+  } else {
+    for (size_t k = i; k < n16; k += stride) d4[k] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  const size_t body_end = head + 4 * n16;
+  for (size_t k = i; k < head + (words - body_end); k += stride) {
+    const size_t w = k < head ? k : body_end + (k - head);
+    dst[w] = src ? src[w] : 0u;
+  }
 }
 
 __global__ __launch_bounds__(256) void fill_work_kernel(const ViewDesc* __restrict__ views, uint32_t num_views,
@@ -562,17 +657,23 @@ void launch_loss_grad(hipStream_t s, const ViewDesc* views, const ViewDesc* host
   hipLaunchKernelGGL(loss_grad_kernel, grid, dim3(256), 0, s, views, image, target, coef, lambda, grad);
 }
 
-void launch_zero(hipStream_t s, void* p, size_t bytes) {
-  if (bytes == 0) return;
-  // Device buffers from gsr_alloc are 256-byte aligned; bytes is a multiple of 4.
-  size_t n16 = bytes / 16;
-  uint32_t ntail = uint32_t((bytes % 16) / 4);
-  uint32_t* tail = reinterpret_cast<uint32_t*>(static_cast<char*>(p) + n16 * 16);
-  size_t blocks = (n16 + 255) / 256;
+void launch_copy(hipStream_t s, void* dst, const void* src, size_t bytes) {
+  // dst and src are 4-byte aligned and bytes is a multiple of 4.
+  const size_t words = bytes / 4;
+  if (words == 0) return;
+  const uintptr_t d = reinterpret_cast<uintptr_t>(dst);
+  size_t head = ((16 - d % 16) % 16) / 4;
+  if (src && (d - reinterpret_cast<uintptr_t>(src)) % 16 != 0) head = words;  // no common 16-byte phase
+  if (head > words) head = words;
+  const size_t n16 = (words - head) / 4;
+  const size_t scalar = words - 4 * n16;
+  size_t blocks = ((n16 > scalar ? n16 : scalar) + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   if (blocks == 0) blocks = 1;
-  hipLaunchKernelGGL(zero_kernel, dim3(uint32_t(blocks)), dim3(256), 0, s, static_cast<uint4*>(p), n16, tail,
-                     ntail);
+  hipLaunchKernelGGL(copy_words_kernel, dim3(uint32_t(blocks)), dim3(256), 0, s, static_cast<uint32_t*>(dst),
+                     static_cast<const uint32_t*>(src), words, head, n16);
 }
+
+void launch_zero(hipStream_t s, void* p, size_t bytes) { launch_copy(s, p, nullptr, bytes); }
 
 }  // namespace gsr

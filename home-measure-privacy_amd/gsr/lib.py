@@ -21,7 +21,7 @@ GSR_E_CAPACITY = -4
 # Every entry point declared in include/gsr.h.
 EXPORTS = (
     "gsr_create", "gsr_destroy", "gsr_last_error", "gsr_alloc", "gsr_free", "gsr_upload", "gsr_download",
-    "gsr_clear", "gsr_synchronize", "gsr_project", "gsr_bin_sort", "gsr_forward", "gsr_loss_grad", "gsr_backward",
+    "gsr_clear", "gsr_copy", "gsr_synchronize", "gsr_project", "gsr_bin_sort", "gsr_forward", "gsr_loss_grad", "gsr_backward",
     "gsr_shape", "gsr_adam", "gsr_stats", "gsr_batch_create", "gsr_batch_destroy", "gsr_batch_set_cameras",
     "gsr_batch_step", "gsr_batch_entries", "gsr_batch_configure", "gsr_batch_autotune", "gsr_batch_profile",
     "gsr_batch_set_passes",
@@ -70,6 +70,7 @@ def load():
         "gsr_upload": (i32, [P, P, P, sz]),
         "gsr_download": (i32, [P, P, P, sz]),
         "gsr_clear": (i32, [P, P, sz]),
+        "gsr_copy": (i32, [P, P, P, sz]),
         "gsr_synchronize": (i32, [P]),
         "gsr_project": (i32, [P, ctypes.POINTER(Camera), P, P, i32, u32, P]),
         "gsr_bin_sort": (i32, [P, ctypes.POINTER(Camera), P, u32, P, u64, P, ctypes.POINTER(u64)]),
@@ -171,8 +172,14 @@ class Context:
             self._check(self.lib.gsr_download(self.h, out.ctypes.data_as(P), buf.p, out.nbytes))
         return out
 
-    def clear(self, buf: Buffer, nbytes: int | None = None):
-        self._check(self.lib.gsr_clear(self.h, buf.p, buf.nbytes if nbytes is None else nbytes))
+    def clear(self, buf: Buffer, nbytes: int | None = None, offset: int = 0):
+        n = buf.nbytes - offset if nbytes is None else nbytes
+        self._check(self.lib.gsr_clear(self.h, ctypes.c_void_p(buf.ptr + offset), n))
+
+    def copy(self, dst: Buffer, src: Buffer, nbytes: int, dst_offset: int = 0, src_offset: int = 0):
+        """Device-to-device copy queued on the context's stream."""
+        self._check(self.lib.gsr_copy(self.h, ctypes.c_void_p(dst.ptr + dst_offset),
+                                      ctypes.c_void_p(src.ptr + src_offset), nbytes))
 
     def sync(self):
         self._check(self.lib.gsr_synchronize(self.h))

@@ -82,8 +82,12 @@ int gsr_free(gsr_ctx* ctx, void* dptr);
 int gsr_upload(gsr_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* Device -> host copy; waits for all queued work first, as read() (gpu.js:716-726). */
 int gsr_download(gsr_ctx* ctx, void* dst, const void* src, size_t bytes);
-/* Zero bytes of a device buffer, as clear() (gpu.js:773-777). */
+/* Zero bytes of a device buffer, as clear() (gpu.js:773-777).  dptr and bytes
+ * are multiples of 4, as WebGPU's clearBuffer requires. */
 int gsr_clear(gsr_ctx* ctx, void* dptr, size_t bytes);
+/* Device -> device copy of non-overlapping regions, as copyBufferToBuffer
+ * (gpu.js:719); pointers and bytes are multiples of 4.  Queued on the stream. */
+int gsr_copy(gsr_ctx* ctx, void* dst, const void* src, size_t bytes);
 int gsr_synchronize(gsr_ctx* ctx);
 
 /* ---- operators (one per pipeline of gpu.js:712) */

@@ -98,6 +98,17 @@ def test_bin_sort_bit_exact_cfg0(ctx):
     assert order.size > 30000
 
 
+def test_bin_sort_bit_exact_cfg1_full_size(ctx):
+    """300k SH3 Gaussians at 1280x720: every entry of the ordering matches."""
+    sc = scenes.config(1, with_targets=False)
+    cam = sc.cams[0]
+    splats = gpu_project(ctx, cam, sc.gs, sc.sh, 3)
+    log(f"bin cfg1 n={sc.n} {cam.width}x{cam.height}")
+    order, _ = check_bin_exact(ctx, cam, splats)
+    log(f"  entries {order.size}")
+    assert order.size > 10 * sc.n
+
+
 @pytest.mark.parametrize("w,h,n", [(1280, 720, 60000), (250, 130, 5000), (17, 33, 300), (1, 1, 50)])
 def test_bin_sort_bit_exact_sizes(ctx, w, h, n):
     sc = scenes.config(1, n=n, with_targets=False)
@@ -331,3 +342,33 @@ def test_adam_matches_restatement(ctx):
     np.testing.assert_allclose(m_got, m_ref, rtol=1e-6, atol=1e-9)
     for b in (gsb, gb, shb, mb):
         b.free()
+
+
+# ---------------------------------------------------------------- buffers
+
+
+@pytest.mark.parametrize("dst_off,src_off,nbytes", [(0, 0, 1 << 20), (4, 4, 4004), (8, 0, 65540), (12, 4, 36),
+                                                    (0, 0, 0), (4, 0, 4)])
+def test_copy_and_clear(ctx, dst_off, src_off, nbytes):
+    """Device copies and clears of any 4-byte phase leave the bytes around them alone."""
+    rng = np.random.default_rng(nbytes + dst_off)
+    size = nbytes + 64
+    a = rng.integers(0, 2**32, size // 4, dtype=np.uint32)
+    b = rng.integers(0, 2**32, size // 4, dtype=np.uint32)
+    ab, bb = ctx.upload(a), ctx.upload(b)
+    ctx.copy(bb, ab, nbytes, dst_off, src_off)
+    out = ctx.download(bb, np.uint32, b.shape)
+    want = b.copy()
+    want[dst_off // 4:(dst_off + nbytes) // 4] = a[src_off // 4:(src_off + nbytes) // 4]
+    np.testing.assert_array_equal(out, want)
+    ctx.clear(ab, nbytes, dst_off)
+    out = ctx.download(ab, np.uint32, a.shape)
+    want = a.copy()
+    want[dst_off // 4:(dst_off + nbytes) // 4] = 0
+    np.testing.assert_array_equal(out, want)
+    with pytest.raises(lib.GsrError):
+        ctx.copy(bb, ab, 6)
+    with pytest.raises(lib.GsrError):
+        ctx.copy(ab, ab, 64, 0, 16)  # overlapping regions
+    ab.free()
+    bb.free()
