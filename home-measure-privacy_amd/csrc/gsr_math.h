@@ -218,13 +218,13 @@ __device__ inline void sh_colour(int deg, const float sh[48], const float mean[3
 // float32 record as JavaScript does.  False when the rectangle is empty, which
 // includes every NaN bound (Math.max/Math.min propagate NaN and the loops then
 // run zero times).
-__device__ inline bool tile_rect(float u, float v, float r, uint32_t tiles_x, uint32_t tiles_y, int& x0, int& x1,
-                                 int& y0, int& y1) {
-  double du = u, dv = v, dr = r;
-  double fx0 = floor((du - dr - 0.5) / 16.0);
-  double fx1 = floor((du + dr - 0.5) / 16.0);
-  double fy0 = floor((dv - dr - 0.5) / 16.0);
-  double fy1 = floor((dv + dr - 0.5) / 16.0);
+__device__ inline bool tile_rect(float u, float v, float rx, float ry, uint32_t tiles_x, uint32_t tiles_y, int& x0,
+                                 int& x1, int& y0, int& y1) {
+  double du = u, dv = v, drx = rx, dry = ry;
+  double fx0 = floor((du - drx - 0.5) / 16.0);
+  double fx1 = floor((du + drx - 0.5) / 16.0);
+  double fy0 = floor((dv - dry - 0.5) / 16.0);
+  double fy1 = floor((dv + dry - 0.5) / 16.0);
   if (isnan(fx0) || isnan(fx1) || isnan(fy0) || isnan(fy1)) return false;
   fx0 = fmax(0.0, fx0);
   fx1 = fmin(double(tiles_x) - 1.0, fx1);
@@ -233,6 +233,32 @@ __device__ inline bool tile_rect(float u, float v, float r, uint32_t tiles_x, ui
   if (!(fx0 <= fx1) || !(fy0 <= fy1)) return false;
   x0 = int(fx0); x1 = int(fx1); y0 = int(fy0); y1 = int(fy1);
   return true;
+}
+
+constexpr float kNever = -1e30f;  // extent of an entry that cannot hit any pixel
+
+// Half-extents (ex, ey) of the pixel offsets (dx, dy) at which alphaAt
+// (gpu.js:92-103) can return a hit: inside its square |dx|, |dy| <= r and inside
+// the ellipse where opacity * exp(power) >= 1/255, i.e. -2 power <= 2 ln(255 op)
+// with -2 power = [dx dy] [[A B] [B C]] [dx dy]^T, whose x half-width is
+// sqrt(2 t C / det) and y half-width sqrt(2 t A / det).  The threshold t is raised
+// by 0.01 plus a bound on the float32 rounding of the exponent's terms, det is
+// Kahan's compensated A C - B^2, and the widths are padded, so every pixel outside
+// the box evaluates to a miss in the blend's own arithmetic too and skipping it
+// changes no result.  Opacities below ~1/255 get kNever;
+// degenerate conics fall back to the square.
+__device__ inline float2 hit_extent(float A, float B, float C, float op, float r) {
+  const float r1 = r + 1.0f;
+  const float slack = 0.01f + 1e-6f * (fabsf(A) + 2.0f * fabsf(B) + fabsf(C)) * r1 * r1;
+  const float t = __logf(255.0f * op) + slack;
+  const float bb = B * B;
+  const float det = fmaf(A, C, -bb) - fmaf(B, B, -bb);
+  if (t != t || !(det > 0.0f) || !(A > 0.0f) || !(C > 0.0f)) return make_float2(r, r);
+  if (!(t > 0.0f)) return make_float2(kNever, kNever);
+  const float k = 2.0f * t / det;
+  const float ex = fmaf(sqrtf(k * C), 1.0001f, 1e-3f);
+  const float ey = fmaf(sqrtf(k * A), 1.0001f, 1e-3f);
+  return make_float2(fminf(r, ex), fminf(r, ey));
 }
 
 }  // namespace gsr

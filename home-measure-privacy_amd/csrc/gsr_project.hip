@@ -1,6 +1,8 @@
 // Per-Gaussian projection: PREPROCESS (trainer/gpu.js:64-88) with projectShape
 // (gpu.js:30-60), plus SH colour (extension) and the tile-binning inputs of
 // tileLists (gpu.js:450-481): depth key, float64 tile rectangle, tile count.
+// The single-view binning (prep_kernel) uses the reference's square rectangle;
+// the batched step's projection uses the tighter hit-box rectangle.
 //
 // One thread per Gaussian reads its 64-byte record (and 192 bytes of SH) once as
 // float4s and loops over every view of the launch, so a batch of V views reads
@@ -54,10 +56,13 @@ __device__ inline void write_splat(float* __restrict__ splats, size_t idx, const
   o[2] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f);
 }
 
+// Depth key and tile rectangle of half-widths (rx, ry) around (u, v); splats
+// without tiles get the invalid key.
 __device__ inline void write_bin_inputs(uint32_t* keys, uint2* rects, uint32_t* counts, size_t idx, bool valid,
-                                        float u, float v, float r, float depth, uint32_t tiles_x, uint32_t tiles_y) {
+                                        float u, float v, float rx, float ry, float depth, uint32_t tiles_x,
+                                        uint32_t tiles_y) {
   int x0 = 0, x1 = -1, y0 = 0, y1 = -1;
-  bool any = valid && tile_rect(u, v, r, tiles_x, tiles_y, x0, x1, y0, y1);
+  bool any = valid && tile_rect(u, v, rx, ry, tiles_x, tiles_y, x0, x1, y0, y1);
   uint32_t cnt = any ? uint32_t(x1 - x0 + 1) * uint32_t(y1 - y0 + 1) : 0u;
   if (keys) keys[idx] = any ? __float_as_uint(depth) : kInvalidKey;
   if (rects)
@@ -83,9 +88,20 @@ __global__ __launch_bounds__(256) void project_kernel(const gsr_camera* __restri
     if (deg > 0 && p.ok) sh_colour(deg, g.sh, g.mean, cam, rgb);
     size_t idx = size_t(v) * n + i;
     write_splat(splats, idx, p, g.opacity, rgb);
-    if (keys || rects || counts)
-      write_bin_inputs(keys, rects, counts, idx, p.ok, p.u, p.v, p.radius, p.depth, views[v].tiles_x,
+    if (keys || rects || counts) {
+      // The batched step bins each splat to the tiles of its hit box (where
+      // alpha can reach 1/255, padded by half a pixel for the float32 pixel
+      // offsets) within the reference square: the tiles it leaves out hold no
+      // pixel the splat can hit, so renders and gradients are unchanged.
+      float rx = p.radius, ry = p.radius;
+      if (p.ok && fabsf(p.u) < 0x1p20f && fabsf(p.v) < 0x1p20f) {
+        const float2 e = hit_extent(p.a, p.b, p.c, g.opacity, p.radius);
+        rx = e.x < 0.0f ? e.x : fminf(p.radius, e.x + 0.5f);
+        ry = e.y < 0.0f ? e.y : fminf(p.radius, e.y + 0.5f);
+      }
+      write_bin_inputs(keys, rects, counts, idx, p.ok, p.u, p.v, rx, ry, p.depth, views[v].tiles_x,
                        views[v].tiles_y);
+    }
   }
 }
 
@@ -101,7 +117,7 @@ __global__ __launch_bounds__(256) void prep_kernel(const ViewDesc* __restrict__ 
   const float4* r4 = reinterpret_cast<const float4*>(splats + idx * 12);
   float4 a = r4[0], b = r4[1], c = r4[2];
   bool valid = c.w > 0.5f;
-  write_bin_inputs(keys, rects, nullptr, idx, valid, a.x, a.y, b.z, b.w, views[v].tiles_x, views[v].tiles_y);
+  write_bin_inputs(keys, rects, nullptr, idx, valid, a.x, a.y, b.z, b.z, b.w, views[v].tiles_x, views[v].tiles_y);
 }
 
 }  // namespace

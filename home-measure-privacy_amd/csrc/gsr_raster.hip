@@ -19,6 +19,7 @@
 // swaps, then DPP within rows), and nine lanes issue one float atomic each into
 // the 16-float gradient record.
 #include "gsr_internal.h"
+#include "gsr_math.h"
 
 namespace gsr {
 
@@ -89,37 +90,14 @@ __device__ inline Batch fetch_batch(const Tile& t, uint32_t k0, uint32_t lane) {
   return b;
 }
 
-constexpr float kNever = -1e30f;  // extent of an entry that cannot hit any pixel
-
-// Half-extents (ex, ey) of the pixel offsets (dx, dy) at which alphaAt
-// (gpu.js:92-103) can return a hit: inside its square |dx|, |dy| <= r and inside
-// the ellipse where opacity * exp(power) >= 1/255, i.e. -2 power <= 2 ln(255 op)
-// with -2 power = [dx dy] [[A B] [B C]] [dx dy]^T, whose x half-width is
-// sqrt(2 t C / det) and y half-width sqrt(2 t A / det).  The threshold t is raised
-// by 0.01 plus a bound on the float32 rounding of the exponent's terms, det is
-// Kahan's compensated A C - B^2, and the widths are padded, so every pixel outside
-// the box evaluates to a miss in the blend's own arithmetic too and skipping it
-// changes no result.  Invalid splats and opacities below ~1/255 get kNever;
-// degenerate conics fall back to the square.
-__device__ inline float2 hit_extent(const Batch& b) {
-  const float A = b.r0.z, B = b.r0.w, C = b.r1.x, op = b.r1.y, r = b.r1.z;
-  if (b.r2.w < 0.5f) return make_float2(kNever, kNever);
-  const float r1 = r + 1.0f;
-  const float slack = 0.01f + 1e-6f * (fabsf(A) + 2.0f * fabsf(B) + fabsf(C)) * r1 * r1;
-  const float t = __logf(255.0f * op) + slack;
-  const float bb = B * B;
-  const float det = fmaf(A, C, -bb) - fmaf(B, B, -bb);
-  if (t != t || !(det > 0.0f) || !(A > 0.0f) || !(C > 0.0f)) return make_float2(r, r);
-  if (!(t > 0.0f)) return make_float2(kNever, kNever);
-  const float k = 2.0f * t / det;
-  const float ex = fmaf(sqrtf(k * C), 1.0001f, 1e-3f);
-  const float ey = fmaf(sqrtf(k * A), 1.0001f, 1e-3f);
-  return make_float2(fminf(r, ex), fminf(r, ey));
+__device__ inline float2 entry_extent(const Batch& b) {
+  if (b.r2.w < 0.5f) return make_float2(kNever, kNever);  // invalid splat
+  return hit_extent(b.r0.z, b.r0.w, b.r1.x, b.r1.y, b.r1.z);
 }
 
 // LDS layout of a staged entry: (u, v, A, B), (C, opacity, radius, ex), (r, g, b, ey).
 __device__ inline void stage(float4 (*lds)[3], uint32_t lane, const Batch& b) {
-  const float2 e = hit_extent(b);
+  const float2 e = entry_extent(b);
   lds[lane][0] = b.r0;
   lds[lane][1] = make_float4(b.r1.x, b.r1.y, b.r1.z, e.x);
   lds[lane][2] = make_float4(b.r2.x, b.r2.y, b.r2.z, e.y);

@@ -73,8 +73,13 @@ def test_batch_matches_single_view_path(ctx):
         imgs.append(out["image"].ravel())
         ents.append(out["entries"])
     r.close()
-    np.testing.assert_array_equal(entries, np.array(ents, dtype=np.uint64))
-    # same projection, ordering and blend arithmetic: identical images
+    # The batched step bins each splat to the tiles of its hit box (alpha can
+    # reach 1/255 there) inside the reference square, so its lists are subsets of
+    # the reference lists; the entries left out cannot hit any pixel.
+    ents = np.array(ents, dtype=np.uint64)
+    assert (entries <= ents).all() and entries.sum() < ents.sum()
+    log(f"  entries batched {int(entries.sum())} reference {int(ents.sum())}")
+    # same projection, depth order and blend arithmetic: identical images
     np.testing.assert_array_equal(img_b, np.concatenate(imgs))
     # gradients differ only by float atomic summation order
     assert rel_l2(g_b[:, :4], g_sum[:, :4]) < 1e-5
