@@ -234,6 +234,20 @@ static napi_value js_clear(napi_env env, napi_callback_info info) {
   return undefined(env);
 }
 
+/* copy(ctx, dst, src, bytes) — copyBufferToBuffer gpu.js:719 */
+static napi_value js_copy(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return NULL;
+  gsr_ctx* ctx = get_ctx(env, argv[0]);
+  void *dst, *src;
+  uint64_t bytes;
+  if (!ctx || !get_ptr(env, argv[1], &dst) || !get_ptr(env, argv[2], &src) || !get_u64(env, argv[3], &bytes))
+    return NULL;
+  int rc = gsr_copy(ctx, dst, src, (size_t)bytes);
+  if (rc) return throw_gsr(env, ctx, rc);
+  return undefined(env);
+}
+
 static napi_value js_sync(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
@@ -401,6 +415,7 @@ static napi_value init(napi_env env, napi_value exports) {
   } fns[] = {
       {"create", js_create},     {"alloc", js_alloc},       {"free", js_free},
       {"upload", js_upload},     {"download", js_download}, {"clear", js_clear},
+      {"copy", js_copy},
       {"synchronize", js_sync},  {"project", js_project},   {"binSort", js_bin_sort},
       {"forward", js_forward},   {"lossGrad", js_loss_grad}, {"backward", js_backward},
       {"shape", js_shape},       {"adam", js_adam},         {"stats", js_stats},

@@ -11,7 +11,7 @@
 //
 // Reference citations are trainer/gpu.js:line unless another file is named.
 //
-// Pinning (tests/test_oracle_*.py):
+// Pinning (tests/test_oracle.py):
 //   project      closed forms (isotropic Gaussian, conic*cov = I, q == -q,
 //                near-plane / det rejects), brute-force radius definition.
 //   SH basis     orthonormality by quadrature on the sphere, degree-0 reduction.

@@ -322,6 +322,62 @@ def test_gradients_end_to_end_crop(ctx):
     r.close()
 
 
+def test_gradients_match_central_differences_64_params(ctx):
+    """cfg 2 differentiation check: GPU gradients (SH3, 4k Gaussians, 128x128 crop)
+    against FP64 central differences of the loss on 64 random parameters (means,
+    log-scales, rotations, opacity logits, SH coefficients).  h = 1e-7: at h = 1e-3
+    the differences cross the renderer's discrete rules (ceil'd radius, 1/255
+    skip, tile membership) for many parameters.  A parameter whose difference
+    quotient disagrees with the FP64 analytic gradient crossed such a rule and is
+    not counted; at least 56 of the 64 must count."""
+    sc = scenes.config(2, n=4000)
+    cam = sc.cams[0].crop(896, 476, 128, 128)
+    gs, sh = sc.gs.astype(np.float64), sc.sh.astype(np.float64)
+    w = (np.random.default_rng(9).standard_normal((cam.height, cam.width, 3)) / (cam.pixels * 3))
+    r = Renderer()
+    out = r.gradients(sc.gs, cam, None, sh=sc.sh, deg=3, dl=w.astype(np.float32))
+    r.close()
+    _, grads, shape, shg = O.gradients(cam, gs, w, sh, 3, np.float64)
+
+    def L(g, s):
+        return float((O.render(cam, g, s, 3)[3] * w).sum())
+
+    scale = max(np.abs(shape).max(), np.abs(grads[:, 3]).max(), np.abs(shg).max())
+    rng = np.random.default_rng(64)
+    touched = np.where((np.abs(shape).sum(1) + np.abs(grads[:, 3])) > 0)[0]
+    slots = [0, 1, 2, 4, 5, 6, 8, 9, 10, 11, 7]  # mean, logScale, rotation, opacity logit
+    h = 1e-7
+    counted, worst = 0, 0.0
+    for _ in range(64):
+        i = int(rng.choice(touched))
+        kind = int(rng.integers(0, 12))
+        if kind < 11:
+            s = slots[kind]
+            gp, gm = gs.copy(), gs.copy()
+            gp[i, s] += h
+            gm[i, s] -= h
+            num = (L(gp, sh) - L(gm, sh)) / (2 * h)
+            an64 = shape[i, kind] if kind < 10 else grads[i, 3]
+            gpu = out["shape"][i, kind] if kind < 10 else out["grads"][i, 3]
+        else:
+            kk, c = int(rng.integers(0, 16)), int(rng.integers(0, 3))
+            sp_, sm_ = sh.copy(), sh.copy()
+            sp_[i, kk, c] += h
+            sm_[i, kk, c] -= h
+            num = (L(gs, sp_) - L(gs, sm_)) / (2 * h)
+            an64 = shg[i, kk, c]
+            gpu = out["sh_grad"][i, kk, c]
+        denom = max(abs(num), 1e-3 * scale)
+        if abs(num - an64) / denom > 1e-4:
+            continue  # the difference quotient crossed a discrete rule
+        counted += 1
+        err = abs(float(gpu) - num) / denom
+        worst = max(worst, err)
+        assert err < 1e-3, (i, kind)
+    log(f"  64-parameter FD check: {counted} counted, worst relative error {worst:.2e}")
+    assert counted >= 56
+
+
 def test_adam_matches_restatement(ctx):
     sc = scenes.config(0, n=5000)
     n = sc.n
