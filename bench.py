@@ -219,10 +219,13 @@ def main():
     dom_ms = sort_ms if dominant == "bin_sort" else timed[dominant]
     achieved = bytes_[dominant] / (dom_ms * 1e-3) / 1e9
     bwd_achieved = bytes_["backward"] / (timed["backward"] * 1e-3) / 1e9
+    # HBM bytes per step of the dominant stage's kernel(s), from the rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes committed for this workload (copy-calibrated).
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{dominant}.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+    tpath = os.path.join(ROOT, "profiles", "traffic_batched_1M_32views.json")
+    if (n, V) == (1_000_000, 32) and os.path.exists(tpath):
+        traffic = json.load(open(tpath))["stages"].get(dominant)
+        traffic = None if traffic is None else round(traffic)
 
     plain = None
     if not args.no_plain:

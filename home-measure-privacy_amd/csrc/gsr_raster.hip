@@ -149,13 +149,13 @@ struct Box {
   float x0, x1, y0, y1;
 };
 
-__device__ inline Box active_box(float px, const float py[kPix], const float T[kPix]) {
+__device__ inline Box active_box(float px, float py0, const float T[kPix]) {
   float ylo = __builtin_huge_valf(), yhi = -__builtin_huge_valf();
 #pragma unroll
   for (int i = 0; i < kPix; ++i)
     if (T[i] > kTStop) {
-      ylo = fminf(ylo, py[i]);
-      yhi = fmaxf(yhi, py[i]);
+      ylo = fminf(ylo, py0 + 4.0f * i);
+      yhi = fmaxf(yhi, py0 + 4.0f * i);
     }
   const bool any = ylo <= yhi;
   Box b{any ? px : __builtin_huge_valf(), any ? px : -__builtin_huge_valf(), ylo, yhi};
@@ -195,12 +195,12 @@ __device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) 
   tile_segment(a, a.pass, t);
   const uint32_t x = t.x0 + (lane & 15);
   const float px = float(x) + 0.5f;
-  float py[kPix], T[kPix], cr[kPix], cg[kPix], cb[kPix];
+  const float py0 = float(t.y0 + (lane >> 4)) + 0.5f;  // row of pixel i: py0 + 4 i
+  float T[kPix], cr[kPix], cg[kPix], cb[kPix];
   uint32_t last[kPix];
 #pragma unroll
   for (int i = 0; i < kPix; ++i) {
     uint32_t y = t.y0 + (lane >> 4) + 4 * i;
-    py[i] = float(y) + 0.5f;
     bool inside = x < t.width && y < t.height;
     T[i] = inside ? 1.0f : 0.0f;  // outside pixels count as done
     cr[i] = cg[i] = cb[i] = 0.0f;
@@ -223,11 +223,11 @@ __device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) 
     __syncthreads();
     if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
     const uint32_t nk = min(64u, t.end - k0);
-    Box box = active_box(px, py, T);
+    Box box = active_box(px, py0, T);
     for (uint32_t j = 0; j < nk; ++j) {
       if ((j & 15) == 0 && j != 0) {
         if (!any_active(T)) break;
-        if ((j & 31) == 0) box = active_box(px, py, T);  // a stale box is a superset: still exact
+        if ((j & 31) == 0) box = active_box(px, py0, T);  // a stale box is a superset: still exact
       }
       walked = k0 + j + 1;
       const float4 r0 = lds[j][0], r1 = lds[j][1], r2 = lds[j][2];
@@ -240,7 +240,7 @@ __device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) 
         // A band outside the hit box, a miss or a done pixel blends with weight 0:
         // colour and T unchanged.
         if (band_culled(band0 + 4.0f * i, v, ey)) continue;
-        const float alpha = alpha_at(py[i] - v, q, row, op, rad, T[i]);
+        const float alpha = alpha_at((py0 + 4.0f * i) - v, q, row, op, rad, T[i]);
         const float wgt = alpha * T[i];
         cr[i] = fmaf(r2.x, wgt, cr[i]);
         cg[i] = fmaf(r2.y, wgt, cg[i]);
@@ -368,23 +368,25 @@ __device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3],
   const uint32_t x = t.x0 + (lane & 15);
   const float px = float(x) + 0.5f;
   const int slot = reduce9_slot(lane);
-  // Per pixel: dL/dC (d), sum_c dL/dC_c * C_final_c (DF) and the running
-  // sum_c dL/dC_c * accumulated_c (S); the colour behind an entry, weighted by
-  // dL/dC, is DF - S (gpu.js:266-268 with behind = C_final - accumulated).
-  float py[kPix], T[kPix], S[kPix], DF[kPix], dr[kPix], dg[kPix], db[kPix];
+  // Per pixel: dL/dC (d), the final colour (f, the forward's image) and the
+  // colour accumulated so far (acc, the forward's own fma sequence, so f - acc
+  // is exactly the colour behind an entry and exactly 0 after the last hit).
+  const float py0 = float(t.y0 + (lane >> 4)) + 0.5f;  // row of pixel i: py0 + 4 i
+  float T[kPix], ar[kPix], ag[kPix], ab[kPix], fr[kPix], fg[kPix], fb[kPix], dr[kPix], dg[kPix], db[kPix];
 #pragma unroll
   for (int i = 0; i < kPix; ++i) {
     uint32_t y = t.y0 + (lane >> 4) + 4 * i;
-    py[i] = float(y) + 0.5f;
     bool inside = x < t.width && y < t.height;
     T[i] = inside ? 1.0f : 0.0f;
-    S[i] = DF[i] = dr[i] = dg[i] = db[i] = 0.0f;
+    ar[i] = ag[i] = ab[i] = fr[i] = fg[i] = fb[i] = dr[i] = dg[i] = db[i] = 0.0f;
     if (inside) {
       size_t pix = t.pixel_base + size_t(y) * t.width + x;
+      fr[i] = a.image[pix * 3];
+      fg[i] = a.image[pix * 3 + 1];
+      fb[i] = a.image[pix * 3 + 2];
       dr[i] = a.dl[pix * 3];
       dg[i] = a.dl[pix * 3 + 1];
       db[i] = a.dl[pix * 3 + 2];
-      DF[i] = fmaf(dr[i], a.image[pix * 3], fmaf(dg[i], a.image[pix * 3 + 1], db[i] * a.image[pix * 3 + 2]));
     }
   }
   const float band0 = float(t.y0) + 0.5f;
@@ -401,11 +403,11 @@ __device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3],
     __syncthreads();
     if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
     const uint32_t nk = min(64u, t.end - k0);
-    Box box = active_box(px, py, T);
+    Box box = active_box(px, py0, T);
     for (uint32_t j = 0; j < nk; ++j) {
       if ((j & 15) == 0 && j != 0) {
         if (!any_active(T)) break;
-        box = active_box(px, py, T);
+        box = active_box(px, py0, T);
       }
       const float4 r0 = lds[j][0], r1 = lds[j][1], r2 = lds[j][2];
       const float u = r0.x, v = r0.y, A = r0.z, B = r0.w, C = r1.x, op = r1.y, rad = r1.z;
@@ -422,8 +424,8 @@ __device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3],
 #pragma unroll
       for (int i = 0; i < kPix; ++i) {
         if (band_culled(band0 + 4.0f * i, v, r2.w)) continue;
-        const float dy = py[i] - v;
-        // Same alpha and transmittance as forward_tile, bit for bit.
+        const float dy = (py0 + 4.0f * i) - v;
+        // Same alpha, transmittance and accumulated colour as forward_tile, bit for bit.
         const float alpha = alpha_at(dy, q, row, op, rad, T[i]);
         const bool h = alpha != 0.0f;
         hit |= h;
@@ -431,14 +433,16 @@ __device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3],
         const float wgt = alpha * Tk;
         const float oma = 1.0f - alpha;
         T[i] = Tk * oma;
-        // gpu.js:266-268: colour gradient and dLoss/dalpha.
+        ar[i] = fmaf(r2.x, wgt, ar[i]);
+        ag[i] = fmaf(r2.y, wgt, ag[i]);
+        ab[i] = fmaf(r2.z, wgt, ab[i]);
+        // gpu.js:266-268: colour gradient and dLoss/dalpha = d . (Tk c - behind / (1 - alpha)).
         g[0] = fmaf(dr[i], wgt, g[0]);
         g[1] = fmaf(dg[i], wgt, g[1]);
         g[2] = fmaf(db[i], wgt, g[2]);
         const float dc = fmaf(dr[i], r2.x, fmaf(dg[i], r2.y, db[i] * r2.z));
-        S[i] = fmaf(wgt, dc, S[i]);
-        const float inv = __builtin_amdgcn_rcpf(oma);
-        float dA = fmaf(Tk, dc, -((DF[i] - S[i]) * inv));
+        const float behind = fmaf(dr[i], fr[i] - ar[i], fmaf(dg[i], fg[i] - ag[i], db[i] * (fb[i] - ab[i])));
+        float dA = fmaf(Tk, dc, -(behind * __builtin_amdgcn_rcpf(oma)));
         // gpu.js:270-281: opacity logit and screen (u, v, conic) gradients, only
         // while the 0.99 clamp is not active.
         dA = (h && alpha < kAlphaMax) ? dA : 0.0f;
@@ -459,13 +463,13 @@ __device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3],
   }
 }
 
-__global__ __launch_bounds__(64) void backward_kernel(RasterArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void backward_kernel(RasterArgs a) {
   __shared__ float4 lds[64][3];
   __shared__ uint32_t gi_lds[64];
   backward_tile(a, blockIdx.x, lds, gi_lds);
 }
 
-__global__ __launch_bounds__(64) void backward_persistent_kernel(RasterArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void backward_persistent_kernel(RasterArgs a) {
   __shared__ float4 lds[64][3];
   __shared__ uint32_t gi_lds[64];
   for (;;) {
