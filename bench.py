@@ -152,6 +152,80 @@ def copy_calibration(mib: int):
     ctx.close()
 
 
+def train_loop(args):
+    """Config 4: 3M Gaussians (64 clustered blobs), SH3, 64 views at 1600x1066; each
+    step is the batched forward + backward of all views (L1) followed by the fused
+    SH3 Adam on the summed gradients.  Steps are timed one by one (synchronised) for
+    the p50 / p99 step time."""
+    n = args.gaussians if args.gaussians != 1_000_000 else 3_000_000
+    V = args.views if args.views != 32 else 64
+    log(f"training scene: {n} Gaussians, {V} views of 1600x1066")
+    sc = scenes.config(4, n=n, views=V)
+    ctx = lib.Context(0)
+    gsb, shb = ctx.upload(sc.gs), ctx.upload(sc.sh)
+    tgt = ctx.upload(np.concatenate([t.ravel() for t in sc.targets]).astype(np.float32))
+    sc.targets = None
+    grads, shape, shg = ctx.alloc(n * 64), ctx.alloc(n * 40), ctx.alloc(n * 192)
+    moments = ctx.alloc(n * 480)
+    # lr: means 1.6e-4 and SH DC 5e-3 (the workload's), SH bands 1-3 at DC / 20,
+    # opacity / scale / rotation as trainScene (gpu.js:891)
+    lr = {"color": 5e-3, "sh_rest": 2.5e-4, "mean": 1.6e-4, "opacity": 0.05, "scale": 0.005, "rotation": 0.001}
+    batch = lib.Batch(ctx, sc.cams, n, 3)
+    t = [0]
+
+    def step():
+        batch.step(gsb, shb, tgt, 0.0, grads, shape, shg, None, True)
+        t[0] += 1
+        ctx.adam_sh(float(t[0]), n, lr, gsb, shb, grads, shape, shg, moments)
+
+    step()
+    if not args.no_autotune:
+        tile_batch, chunk = batch.autotune(gsb, shb, tgt, 0.0, grads, shape, shg)
+        log(f"autotune: tile_batch={tile_batch} chunk={chunk}")
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    times = []
+    t_all = time.perf_counter()
+    for k in range(args.train_steps):
+        t0 = time.perf_counter()
+        step()
+        ctx.sync()
+        times.append(time.perf_counter() - t0)
+        if (k + 1) % 50 == 0:
+            log(f"  step {k + 1}: {np.median(times) * 1e3:.1f} ms median")
+    total = time.perf_counter() - t_all
+    stage_ms, counters = batch.profile(gsb, shb, tgt, 0.0, grads, shape, shg)
+    # Adam alone, for its bandwidth: reads gs, sh, grads, shape, sh_grad, moments;
+    # writes gs, sh, moments (bytes per Gaussian)
+    ctx.sync()
+    a0 = time.perf_counter()
+    for _ in range(5):
+        ctx.adam_sh(float(t[0]), n, lr, gsb, shb, grads, shape, shg, moments)
+    ctx.sync()
+    adam_ms = (time.perf_counter() - a0) / 5 * 1e3
+    adam_bytes = n * (64 + 192 + 16 + 40 + 192 + 480 + 64 + 192 + 480)
+    ms = np.array(times) * 1e3
+    E = int(counters[:, 0].sum())
+    line = {
+        "metric": "views/s training step (batched fwd+bwd + fused SH3 Adam), 3M Gaussians, 64 views 1600x1066",
+        "value": round(V * len(times) / total, 3), "unit": "views/s", "n_gpus": 1, "steps": len(times),
+        "warmup": args.warmup, "ms_per_step": round(float(total / len(times) * 1e3), 3),
+        "p50_ms": round(float(np.percentile(ms, 50)), 3), "p99_ms": round(float(np.percentile(ms, 99)), 3),
+        "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{n} Gaussians SH3 in 64 clusters (sigma 0.3) in a 4x4x2 box, {V} views 1600x1066 "
+                               "on a radius-4 Fibonacci sphere, L1 vs seeded targets, Adam on summed gradients",
+                   "gaussians": n, "views": V, "lr": lr, "entries_per_step": E,
+                   "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+                   "adam_ms": round(adam_ms, 4), "adam_GBps": round(adam_bytes / (adam_ms * 1e-3) / 1e9, 1)},
+    }
+    print(json.dumps(line), flush=True)
+    batch.close()
+    for b in (gsb, shb, tgt, grads, shape, shg, moments):
+        b.free()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +241,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0, help="only time a device copy of this many MiB")
+    ap.add_argument("--train", action="store_true", help="the 3M-Gaussian, 64-view training loop instead")
+    ap.add_argument("--train-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design shards views and "
@@ -174,6 +250,9 @@ def main():
         sys.exit(2)
     if args.copy_mib:
         copy_calibration(args.copy_mib)
+        return
+    if args.train:
+        train_loop(args)
         return
 
     n, V = args.gaussians, args.views

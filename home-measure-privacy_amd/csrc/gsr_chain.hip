@@ -244,6 +244,80 @@ __global__ __launch_bounds__(256) void adam_kernel(gsr_step st, float* __restric
   for (int k = 0; k < 4; ++k) b[8 + k] = qv[k] / len;
 }
 
+// ADAM over the 59 numbers of an SH3 Gaussian (extension of gpu.js:411-443): the
+// reference's update (beta 0.9 / 0.999, eps 1e-8, pow bias correction with t as
+// f32) for mean, log-scale, rotation, opacity logit and the 48 SH coefficients,
+// then the quaternion renormalisation.  The SH coefficients are not clamped (the
+// colour clamp of gpu.js:439 applies to the SH0 colour only).  Moments [n][120]:
+// m of parameter k at k, v at 60 + k, parameters ordered mean 0-2, logScale 3-5,
+// rotation 6-9, opacity logit 10, SH 11-58 ([16][3]).
+__device__ inline float adam_update(float g, float lr, float bc1, float bc2, float& m, float& v) {
+  m = 0.9f * m + 0.1f * g;
+  v = 0.999f * v + 0.001f * g * g;
+  return lr * (m / bc1) / (sqrtf(v / bc2) + 1e-8f);
+}
+
+__global__ __launch_bounds__(256) void adam_sh_kernel(gsr_step st, float lr_sh_rest, float* __restrict__ gs,
+                                                      float* __restrict__ sh, const float* __restrict__ grads,
+                                                      const float* __restrict__ shape,
+                                                      const float* __restrict__ sh_grad,
+                                                      float* __restrict__ moments) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.count) return;
+  const float bc1 = 1.0f - powf(0.9f, st.t);
+  const float bc2 = 1.0f - powf(0.999f, st.t);
+  float* b = gs + size_t(i) * 16;
+  float4* m4 = reinterpret_cast<float4*>(moments + size_t(i) * 120);
+  // Parameters 0-10 from the 16-float record (slots 0-2, 4-6, 8-11, 7).
+  {
+    const int slot[11] = {0, 1, 2, 4, 5, 6, 8, 9, 10, 11, 7};
+    float m[12], v[12];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float4 a = m4[q], c = m4[15 + q];
+      m[4 * q] = a.x; m[4 * q + 1] = a.y; m[4 * q + 2] = a.z; m[4 * q + 3] = a.w;
+      v[4 * q] = c.x; v[4 * q + 1] = c.y; v[4 * q + 2] = c.z; v[4 * q + 3] = c.w;
+    }
+    const float* shp = shape + size_t(i) * 10;
+#pragma unroll
+    for (int k = 0; k < 11; ++k) {
+      const float g = k < 10 ? shp[k] : grads[size_t(i) * 16 + 3];
+      const float lr = k < 3 ? st.lr_mean : k < 6 ? st.lr_scale : k < 10 ? st.lr_rotation : st.lr_opacity;
+      b[slot[k]] -= adam_update(g, lr, bc1, bc2, m[k], v[k]);
+    }
+    // parameter 11 (SH 0, red DC) shares the third float4 with 8-10
+    const float g11 = sh_grad[size_t(i) * 48];
+    const float u11 = adam_update(g11, st.lr_color, bc1, bc2, m[11], v[11]);
+    sh[size_t(i) * 48] -= u11;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      m4[q] = make_float4(m[4 * q], m[4 * q + 1], m[4 * q + 2], m[4 * q + 3]);
+      m4[15 + q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    }
+  }
+  // SH parameters 12-58 (coefficients 1..47 of the [16][3] block).
+  float* s = sh + size_t(i) * 48;
+  const float* gsh = sh_grad + size_t(i) * 48;
+  for (int q = 3; q < 15; ++q) {
+    float4 a = m4[q], c = m4[15 + q];
+    float m[4] = {a.x, a.y, a.z, a.w}, v[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = 4 * q + e;  // parameter index; SH coefficient k - 11
+      if (k > 58) break;
+      const int c3 = k - 11;
+      const float lr = c3 < 3 ? st.lr_color : lr_sh_rest;  // DC: lr_color, bands 1-3: lr_sh_rest
+      s[c3] -= adam_update(gsh[c3], lr, bc1, bc2, m[e], v[e]);
+    }
+    m4[q] = make_float4(m[0], m[1], m[2], m[3]);
+    m4[15 + q] = make_float4(v[0], v[1], v[2], v[3]);
+  }
+  float qv[4] = {b[8], b[9], b[10], b[11]};
+  float len = fmaxf(sqrtf(qv[0] * qv[0] + qv[1] * qv[1] + qv[2] * qv[2] + qv[3] * qv[3]), 1e-12f);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) b[8 + k] = qv[k] / len;
+}
+
 // STATS.main (gpu.js:491-497).
 __global__ __launch_bounds__(256) void stats_kernel(uint32_t n, const float* __restrict__ grads,
                                                     const float* __restrict__ splats, float* __restrict__ st) {
@@ -268,6 +342,13 @@ void launch_adam(hipStream_t s, const gsr_step& step, float* gs, const float* gr
                  float* moments) {
   if (step.count == 0) return;
   hipLaunchKernelGGL(adam_kernel, dim3((step.count + 255) / 256), dim3(256), 0, s, step, gs, grads, shape, moments);
+}
+
+void launch_adam_sh(hipStream_t s, const gsr_step& step, float lr_sh_rest, float* gs, float* sh, const float* grads,
+                    const float* shape, const float* sh_grad, float* moments) {
+  if (step.count == 0) return;
+  hipLaunchKernelGGL(adam_sh_kernel, dim3((step.count + 255) / 256), dim3(256), 0, s, step, lr_sh_rest, gs, sh, grads,
+                     shape, sh_grad, moments);
 }
 
 void launch_stats(hipStream_t s, const gsr_step& step, const float* grads, const float* splats, float* stats) {

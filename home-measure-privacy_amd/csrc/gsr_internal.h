@@ -145,6 +145,8 @@ void launch_zero(hipStream_t s, void* p, size_t bytes);
 void launch_shape(hipStream_t s, const gsr_camera* cams, uint32_t num_views, const float* gs, const float* sh,
                   int sh_degree, uint32_t n, float* grads, uint64_t grads_view_stride, float* shape,
                   float* sh_grad, float* grads_out, int consume, uint32_t* touched = nullptr);
+void launch_adam_sh(hipStream_t s, const gsr_step& step, float lr_sh_rest, float* gs, float* sh, const float* grads,
+                    const float* shape, const float* sh_grad, float* moments);
 void launch_adam(hipStream_t s, const gsr_step& step, float* gs, const float* grads, const float* shape,
                  float* moments);
 void launch_stats(hipStream_t s, const gsr_step& step, const float* grads, const float* splats, float* stats);

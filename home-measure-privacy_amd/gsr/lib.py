@@ -22,7 +22,7 @@ GSR_E_CAPACITY = -4
 EXPORTS = (
     "gsr_create", "gsr_destroy", "gsr_last_error", "gsr_alloc", "gsr_free", "gsr_upload", "gsr_download",
     "gsr_clear", "gsr_copy", "gsr_synchronize", "gsr_project", "gsr_bin_sort", "gsr_forward", "gsr_loss_grad", "gsr_backward",
-    "gsr_shape", "gsr_adam", "gsr_stats", "gsr_batch_create", "gsr_batch_destroy", "gsr_batch_set_cameras",
+    "gsr_shape", "gsr_adam", "gsr_adam_sh", "gsr_stats", "gsr_batch_create", "gsr_batch_destroy", "gsr_batch_set_cameras",
     "gsr_batch_step", "gsr_batch_entries", "gsr_batch_configure", "gsr_batch_autotune", "gsr_batch_profile",
     "gsr_batch_set_passes",
 )
@@ -79,6 +79,7 @@ def load():
         "gsr_backward": (i32, [P, ctypes.POINTER(Camera), P, P, P, P, P, P]),
         "gsr_shape": (i32, [P, ctypes.POINTER(Camera), P, P, i32, u32, P, P, P]),
         "gsr_adam": (i32, [P, ctypes.POINTER(Step), P, P, P, P]),
+        "gsr_adam_sh": (i32, [P, ctypes.POINTER(Step), f32, P, P, P, P, P, P]),
         "gsr_stats": (i32, [P, ctypes.POINTER(Step), P, P, P]),
         "gsr_batch_create": (i32, [P, u32, ctypes.POINTER(Camera), u32, i32, ctypes.POINTER(P)]),
         "gsr_batch_destroy": (None, [P]),
@@ -226,6 +227,14 @@ class Context:
     def adam(self, t: float, n: int, lr: dict, gs: Buffer, grads: Buffer, shape: Buffer, moments: Buffer):
         st = Step(t, n, lr["color"], lr["opacity"], lr["mean"], lr["scale"], lr["rotation"], 0.0)
         self._check(self.lib.gsr_adam(self.h, ctypes.byref(st), gs.p, grads.p, shape.p, moments.p))
+
+    def adam_sh(self, t: float, n: int, lr: dict, gs: Buffer, sh: Buffer, grads: Buffer, shape: Buffer,
+                sh_grad: Buffer, moments: Buffer):
+        """Adam over mean, log-scale, rotation, opacity and the 48 SH coefficients;
+        lr["color"] is the SH DC rate and lr["sh_rest"] that of bands 1-3."""
+        st = Step(t, n, lr["color"], lr["opacity"], lr["mean"], lr["scale"], lr["rotation"], 0.0)
+        self._check(self.lib.gsr_adam_sh(self.h, ctypes.byref(st), lr["sh_rest"], gs.p, sh.p, grads.p, shape.p,
+                                         sh_grad.p, moments.p))
 
     def stats(self, n: int, grads: Buffer, splats: Buffer, stats: Buffer):
         st = Step(0.0, n, 0, 0, 0, 0, 0, 0)

@@ -140,6 +140,17 @@ int gsr_shape(gsr_ctx* ctx, const gsr_camera* cam, const float* gs, const float*
 int gsr_adam(gsr_ctx* ctx, const gsr_step* step, float* gs, const float* grads, const float* shape,
              float* moments);
 
+/* ADAM over the 59 numbers of an SH3 Gaussian (extension of gpu.js:411-443, for
+ * SH training): mean (lr_mean), log-scale (lr_scale), rotation (lr_rotation),
+ * opacity logit (lr_opacity, gradient grads[3]), SH DC (lr_color) and SH bands
+ * 1-3 (lr_sh_rest), with the reference's update and quaternion renormalisation;
+ * SH coefficients are not clamped.  gs, sh [n][16][3] and moments [n][120]
+ * (m of parameter k at k, v at 60 + k; parameters mean 0-2, logScale 3-5,
+ * rotation 6-9, opacity 10, SH 11-58) are updated in place; sh_grad as written
+ * by gsr_shape / gsr_batch_step.  sh and moments are 16-byte aligned. */
+int gsr_adam_sh(gsr_ctx* ctx, const gsr_step* step, float lr_sh_rest, float* gs, float* sh, const float* grads,
+                const float* shape, const float* sh_grad, float* moments);
+
 /* STATS (gpu.js:483-497): stats[2i] += |(du, dv)|, stats[2i+1] += 1 for valid splats. */
 int gsr_stats(gsr_ctx* ctx, const gsr_step* step, const float* grads, const float* splats, float* stats);
 

@@ -23,7 +23,7 @@
 //   backward     central differences of the FP64 forward (weighted-sum loss).
 //   shape chain  central differences of FP64 projectShape, and of the full FP64
 //                forward with respect to every parameter group.
-//   adam         Kingma & Ba closed form at t = 1 and t = 2.
+//   adam         Kingma & Ba closed form at t = 1 and t = 2 (also the SH3 variant).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -730,6 +730,48 @@ void shape_fd(const CamU* cu, const R* gs, uint32_t n, const R* grads, R* shape)
   }
 }
 
+// Adam over the 59 numbers of an SH3 Gaussian (the extension of gpu.js:411-443):
+// parameters mean 0-2, logScale 3-5, rotation 6-9, opacity logit 10, SH 11-58;
+// moments [n][120] with m at k and v at 60 + k.  The reference's update per
+// parameter, then the quaternion renormalisation; SH coefficients unclamped.
+template <class R>
+void adam_sh(const StepU* st, float lr_sh_rest, float* gs, float* sh, const R* grads, const R* shape,
+             const R* sh_grad, R* moments) {
+  static const int SLOT[11] = {0, 1, 2, 4, 5, 6, 8, 9, 10, 11, 7};
+  const uint32_t n = st->count;
+  const R t = R(st->t);
+  const R bc1 = R(1) - std::pow(R(0.9), t), bc2 = R(1) - std::pow(R(0.999), t);
+  for (uint32_t i = 0; i < n; ++i) {
+    R* mo = moments + size_t(i) * 120;
+    for (int k = 0; k < 59; ++k) {
+      R g, lr;
+      float* p;
+      if (k < 10) {
+        g = shape[i * 10 + k];
+        lr = R(k < 3 ? st->lrMean : k < 6 ? st->lrScale : st->lrRotation);
+        p = gs + i * 16 + SLOT[k];
+      } else if (k == 10) {
+        g = grads[i * 16 + 3];
+        lr = R(st->lrOpacity);
+        p = gs + i * 16 + 7;
+      } else {
+        g = sh_grad[i * 48 + (k - 11)];
+        lr = R(k - 11 < 3 ? st->lrColor : lr_sh_rest);
+        p = sh + i * 48 + (k - 11);
+      }
+      R m = R(0.9) * mo[k] + R(0.1) * g;
+      R v = R(0.999) * mo[60 + k] + R(0.001) * g * g;
+      mo[k] = m;
+      mo[60 + k] = v;
+      *p = float(R(*p) - lr * (m / bc1) / (std::sqrt(v / bc2) + R(1e-8)));
+    }
+    float* b = gs + i * 16;
+    R qv[4] = {R(b[8]), R(b[9]), R(b[10]), R(b[11])};
+    R len = std::max(std::sqrt(qv[0] * qv[0] + qv[1] * qv[1] + qv[2] * qv[2] + qv[3] * qv[3]), R(1e-12));
+    for (int k = 0; k < 4; ++k) b[8 + k] = float(qv[k] / len);
+  }
+}
+
 // ADAM.main, gpu.js:422-443 (14 parameters, SLOT map gpu.js:420).  gs and moments
 // are updated in place.
 template <class R>
@@ -820,6 +862,10 @@ int oracle_num_threads(int n) {
   }                                                                                                                   \
   void oracle_adam_##SUF(const StepU* st, float* gs, const R* grads, const R* shape, R* moments) {                  \
     adam<R>(st, gs, grads, shape, moments);                                                                          \
+  }                                                                                                                   \
+  void oracle_adam_sh_##SUF(const StepU* st, float lr_sh_rest, float* gs, float* sh, const R* grads,               \
+                            const R* shape, const R* sh_grad, R* moments) {                                          \
+    adam_sh<R>(st, lr_sh_rest, gs, sh, grads, shape, sh_grad, moments);                                              \
   }                                                                                                                   \
   void oracle_stats_##SUF(uint32_t n, const R* grads, const R* splats, R* st) { stats<R>(n, grads, splats, st); }   \
   void oracle_sh_basis_##SUF(int deg, const R* d, R* Y) { sh_basis<R>(deg, d, Y); }                                  \

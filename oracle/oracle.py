@@ -178,6 +178,18 @@ def adam(t, lr, gs, grads, shape, moments, dtype=np.float32):
     return gs, moments
 
 
+def adam_sh(t, lr, lr_sh_rest, gs, sh, grads, shape, sh_grad, moments, dtype=np.float32):
+    """Adam over the 59 numbers of an SH3 Gaussian; returns (gs, sh, moments)."""
+    gs = _c(gs, np.float32).copy()
+    sh = _c(sh, np.float32).copy()
+    moments = _c(moments, dtype).copy()
+    n = gs.shape[0]
+    getattr(lib(), f"oracle_adam_sh_{_suf(dtype)}")(step_buf(t, n, lr), ctypes.c_float(lr_sh_rest), _p(gs), _p(sh),
+                                                    _p(_c(grads, dtype)), _p(_c(shape, dtype)),
+                                                    _p(_c(sh_grad, dtype)), _p(moments))
+    return gs, sh, moments
+
+
 def stats(grads, splats, st, dtype=np.float32):
     st = _c(st, dtype).copy()
     n = splats.shape[0]

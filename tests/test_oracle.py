@@ -411,6 +411,36 @@ def test_adam_closed_form():
     np.testing.assert_allclose(mom2[:, 14:], v, rtol=1e-12)
 
 
+def test_adam_sh_closed_form():
+    """SH3 Adam: at t = 1 every parameter moves by lr * g / (|g| + 1e-8) (Kingma & Ba
+    with m = 0.1 g, v = 0.001 g^2 and bias corrections 0.1, 0.001), each group at
+    its own rate; then the quaternion is renormalised."""
+    rng = np.random.default_rng(7)
+    n = 50
+    gs = rng.standard_normal((n, 16)).astype(np.float32)
+    sh = rng.standard_normal((n, 16, 3)).astype(np.float32)
+    grads = rng.standard_normal((n, 16))
+    shape = rng.standard_normal((n, 10))
+    shg = rng.standard_normal((n, 16, 3))
+    lr = {"color": 5e-3, "opacity": 0.05, "mean": 1.6e-4, "scale": 5e-3, "rotation": 1e-3}
+    gs1, sh1, mom = O.adam_sh(1.0, lr, 2.5e-4, gs, sh, grads, shape, shg, np.zeros((n, 120)), np.float64)
+    step = lambda g, r: r * g / (np.abs(g) + 1e-8)  # noqa: E731
+    np.testing.assert_allclose(gs1[:, 0:3], gs[:, 0:3] - step(shape[:, 0:3], lr["mean"]), atol=1e-6)
+    np.testing.assert_allclose(gs1[:, 4:7], gs[:, 4:7] - step(shape[:, 3:6], lr["scale"]), atol=1e-6)
+    np.testing.assert_allclose(gs1[:, 7], gs[:, 7] - step(grads[:, 3], lr["opacity"]), atol=1e-6)
+    q = gs[:, 8:12] - step(shape[:, 6:10], lr["rotation"])
+    np.testing.assert_allclose(gs1[:, 8:12], q / np.linalg.norm(q, axis=1, keepdims=True), atol=1e-6)
+    rates = np.full((16, 3), 2.5e-4)
+    rates[0] = lr["color"]
+    np.testing.assert_allclose(sh1, sh - step(shg, rates[None]), atol=1e-6)
+    np.testing.assert_allclose(mom[:, 10], 0.1 * grads[:, 3], rtol=1e-12)
+    np.testing.assert_allclose(mom[:, 11:59], 0.1 * shg.reshape(n, 48), rtol=1e-12)
+    np.testing.assert_allclose(mom[:, 71:119], 0.001 * shg.reshape(n, 48) ** 2, rtol=1e-12)
+    assert not mom[:, 59].any() and not mom[:, 119].any()
+    # untouched slots: pad 3, colour 12-14 (SH0 colour is not an SH3 parameter), 15
+    np.testing.assert_array_equal(gs1[:, [3, 12, 13, 14, 15]], gs[:, [3, 12, 13, 14, 15]])
+
+
 def test_stats_accumulate():
     sp = np.zeros((3, 12), np.float32)
     sp[:, 11] = [1, 0, 1]
