@@ -60,6 +60,15 @@ struct gsr_batch {
   uint32_t* ranges = nullptr;   // [kMaxPasses][total_tiles][2]
   float* tstate = nullptr;      // per pixel transmittance between forward passes
   uint8_t* done = nullptr;      // per tile: saturated
+  uint8_t* band_open = nullptr; // per tile, 4 bytes: band still open after the last forward pass
+  uint32_t* walk = nullptr;     // [kMaxPasses][total_tiles]: entries the forward examined per pass
+  TileWork* heavy = nullptr;    // [4 total_tiles]: band items of the split tiles of a launch
+  uint8_t* split = nullptr;     // [total_tiles]: tile split in the current launch
+  // Tiles are blended one band per wave when their cost exceeds these (0: never):
+  // the forward of the first depth slice (most tiles saturate early, so list
+  // length says little about the walk), the forward of later slices (unsaturated
+  // tiles walk their whole segment) and the backward (entries the forward walked).
+  uint32_t split_first = 0, split_later = 0, split_backward = 1024;
   uint32_t* order = nullptr;
   uint64_t order_cap = 0;
   uint32_t* flags = nullptr;  // [0] overflow, [1 + p] forward queue of pass p, [1 + kMaxPasses] backward queue
@@ -144,6 +153,9 @@ int alloc_all(gsr_batch* b) {
       (rc = dalloc(b, &b->ebase, kMaxPasses * b->V)) || (rc = dalloc(b, &b->run, 1)) ||
       (rc = dalloc(b, &b->ranges, size_t(kMaxPasses) * b->total_tiles * 2)) ||
       (rc = dalloc(b, &b->tstate, b->total_pixels)) || (rc = dalloc(b, &b->done, b->total_tiles)) ||
+      (rc = dalloc(b, &b->band_open, size_t(b->total_tiles) * 4)) ||
+      (rc = dalloc(b, &b->walk, size_t(kMaxPasses) * b->total_tiles)) ||
+      (rc = dalloc(b, &b->heavy, size_t(b->total_tiles) * 4)) || (rc = dalloc(b, &b->split, b->total_tiles)) ||
       (rc = dalloc(b, &b->flags, 64)) || (rc = dalloc(b, &b->image, b->total_pixels * 3)) ||
       (rc = dalloc(b, &b->dl, b->total_pixels * 3)) ||
       (rc = dalloc(b, &b->vgrads, size_t(b->V) * std::max<uint32_t>(b->n, 1) * 16)))
@@ -189,8 +201,7 @@ int upload_cams(gsr_batch* b) {
 struct Prof {
   hipEvent_t ev[7];  // boundaries around project, sort, binning + forward, loss, backward, chain
   float binned_ms[3];             // bin count, bin emit, forward (summed over passes)
-  uint32_t* tile_last;
-  uint32_t* tile_walk;
+  uint32_t* last;                 // optional, [kMaxPasses][total_tiles]: furthest last hit - segment start
   uint32_t* touched;
 };
 
@@ -204,6 +215,8 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
     if (prof) hipEventRecord(prof->ev[stage++], s);
   };
   launch_zero(s, b->flags, 64 * sizeof(uint32_t));
+  launch_zero(s, b->walk, size_t(b->num_passes) * b->total_tiles * sizeof(uint32_t));
+  if (prof && prof->last) launch_zero(s, prof->last, size_t(b->num_passes) * b->total_tiles * sizeof(uint32_t));
   mark();
   launch_project(s, b->d_cams, b->d_views, b->V, gs, sh, b->deg, b->n, b->splats, b->keys, b->rects, nullptr);
   mark();
@@ -231,8 +244,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   a.tile_batch = b->tile_batch;
   a.tstate = b->num_passes > 1 ? b->tstate : nullptr;
   a.done = b->num_passes > 1 ? b->done : nullptr;
-  a.tile_last = prof ? prof->tile_last : nullptr;
-  a.tile_walk = prof ? prof->tile_walk : nullptr;
+  a.band_open = b->num_passes > 1 ? b->band_open : nullptr;
   // Binning and forward alternate per depth slice: a slice is binned only for
   // the tiles the slices before it left unsaturated.  Profiled steps time all
   // binning as "bin count" + "bin emit" and all forward passes as "forward".
@@ -257,7 +269,17 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
     if (prof) hipEventRecord(e2, s);
     a.pass = int(p);
     a.queue = b->flags + 1 + p;
+    const uint32_t split_cost = p ? b->split_later : b->split_first;
+    a.split = split_cost ? b->split : nullptr;
+    a.heavy = b->heavy;
+    a.heavy_count = b->flags + 16 + p;
+    if (split_cost)
+      launch_split_list(s, SplitArgs{b->d_work, b->total_tiles, split_cost, a.ranges[p], nullptr, 0, b->split,
+                                     b->heavy, b->flags + 16 + p});
+    a.walk = b->walk + size_t(p) * b->total_tiles;
+    a.last = prof && prof->last ? prof->last + size_t(p) * b->total_tiles : nullptr;
     launch_forward(s, a, 1, b->grid_fwd);
+    if (a.done && p + 1 < b->num_passes) launch_done_update(s, b->total_tiles, b->band_open, b->done, p == 0);
     if (prof) {
       hipEventRecord(e3, s);
       hipEventSynchronize(e3);
@@ -276,16 +298,21 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
     prof->binned_ms[2] = t_fwd;
     for (hipEvent_t e : {e0, e1, e2, e3}) hipEventDestroy(e);
   }
-  a.tile_last = nullptr;
-  a.tile_walk = nullptr;
+  a.walk = nullptr;
+  a.last = nullptr;
   mark();
   launch_loss_grad(s, b->d_views, b->views.data(), b->V, image, targets, lambda, b->coef, b->dl);
   mark();
+  a.split = b->split_backward ? b->split : nullptr;
+  a.heavy_count = b->flags + 16 + kMaxPasses;
+  if (b->split_backward)
+    launch_split_list(s, SplitArgs{b->d_work, b->total_tiles, b->split_backward, nullptr, b->walk,
+                                   int(b->num_passes), b->split, b->heavy, b->flags + 16 + kMaxPasses});
   a.queue = b->flags + 1 + kMaxPasses;
   launch_backward(s, a, 1, b->grid_bwd);
   mark();
   launch_shape(s, b->d_cams, b->V, gs, sh, b->deg, b->n, b->vgrads, uint64_t(b->n) * 16, shape, sh_grad, grads, 1,
-               prof ? prof->touched : nullptr);
+               prof && prof->last ? prof->touched : nullptr);
   mark();
 }
 
@@ -399,7 +426,8 @@ void gsr_batch_destroy(gsr_batch* b) {
   drop_graph(b);
   void* ptrs[] = {b->d_cams, b->d_views, b->d_work, b->splats, b->keys, b->keys_alt, b->vals, b->vals_alt,
                   b->rects, b->rect_sorted, b->hist, b->piece, b->matrix, b->tile_tot, b->entries, b->ebase,
-                  b->run, b->ranges, b->tstate, b->done, b->order, b->flags, b->image, b->dl, b->coef, b->vgrads};
+                  b->run, b->ranges, b->tstate, b->done, b->order, b->flags, b->image, b->dl, b->coef, b->vgrads,
+                  b->band_open, b->walk, b->heavy, b->split};
   for (void* p : ptrs) dfree(p);
   if (b->pinned) hipHostFree(b->pinned);
   delete b;
@@ -467,41 +495,67 @@ int gsr_batch_profile(gsr_batch* b, const float* gs, const float* sh, const floa
   hipStream_t s = ctx_stream(b->ctx);
   Prof prof{};
   const size_t nt = std::max<size_t>(1, b->total_tiles);
-  std::vector<uint32_t> tl(nt), tw(nt), touched(b->V);
-  if (hipMalloc(reinterpret_cast<void**>(&prof.tile_last), nt * 4) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void**>(&prof.tile_walk), nt * 4) != hipSuccess ||
+  const size_t np = size_t(b->num_passes) * nt;
+  std::vector<uint32_t> tl(np), tw(np), touched(b->V);
+  if (hipMalloc(reinterpret_cast<void**>(&prof.last), size_t(kMaxPasses) * nt * 4) != hipSuccess ||
       hipMalloc(reinterpret_cast<void**>(&prof.touched), b->V * 4) != hipSuccess) {
-    dfree(prof.tile_last);
-    dfree(prof.tile_walk);
+    dfree(prof.last);
     return ctx_fail(b->ctx, GSR_E_NOMEM, "profile buffers");
   }
   launch_zero(s, prof.touched, b->V * 4);
   for (auto& e : prof.ev) hipEventCreate(&e);
+  // Stage times come from a step that runs the same kernels as a regular step;
+  // the counters from a second one that also tracks each tile's last hit.
+  uint32_t* last = prof.last;
+  prof.last = nullptr;
   issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, &prof);
   hipError_t e = hipStreamSynchronize(s);
+  float binned[3] = {prof.binned_ms[0], prof.binned_ms[1], prof.binned_ms[2]};
+  float marks[6] = {0, 0, 0, 0, 0, 0};
+  if (e == hipSuccess) {
+    hipEventElapsedTime(marks + 0, prof.ev[0], prof.ev[1]);
+    hipEventElapsedTime(marks + 1, prof.ev[1], prof.ev[2]);
+    hipEventElapsedTime(marks + 2, prof.ev[3], prof.ev[4]);
+    hipEventElapsedTime(marks + 3, prof.ev[4], prof.ev[5]);
+    hipEventElapsedTime(marks + 4, prof.ev[5], prof.ev[6]);
+    prof.last = last;
+    launch_zero(s, prof.touched, b->V * 4);
+    issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, &prof);
+    e = hipStreamSynchronize(s);
+  }
   if (e == hipSuccess) {
     // Stage order: project, depth sort, bin count, bin emit, forward, loss, backward, chain.
-    hipEventElapsedTime(stage_ms + 0, prof.ev[0], prof.ev[1]);
-    hipEventElapsedTime(stage_ms + 1, prof.ev[1], prof.ev[2]);
-    stage_ms[2] = prof.binned_ms[0];
-    stage_ms[3] = prof.binned_ms[1];
-    stage_ms[4] = prof.binned_ms[2];
-    hipEventElapsedTime(stage_ms + 5, prof.ev[3], prof.ev[4]);
-    hipEventElapsedTime(stage_ms + 6, prof.ev[4], prof.ev[5]);
-    hipEventElapsedTime(stage_ms + 7, prof.ev[5], prof.ev[6]);
+    stage_ms[0] = marks[0];
+    stage_ms[1] = marks[1];
+    stage_ms[2] = binned[0];
+    stage_ms[3] = binned[1];
+    stage_ms[4] = binned[2];
+    stage_ms[5] = marks[2];
+    stage_ms[6] = marks[3];
+    stage_ms[7] = marks[4];
     if (counters) {
-      hipMemcpy(tl.data(), prof.tile_last, nt * 4, hipMemcpyDeviceToHost);
-      hipMemcpy(tw.data(), prof.tile_walk, nt * 4, hipMemcpyDeviceToHost);
+      hipMemcpy(tl.data(), prof.last, np * 4, hipMemcpyDeviceToHost);
+      hipMemcpy(tw.data(), b->walk, np * 4, hipMemcpyDeviceToHost);
       hipMemcpy(touched.data(), prof.touched, b->V * 4, hipMemcpyDeviceToHost);
       std::vector<uint64_t> pe(size_t(b->num_passes) * b->V), ent(b->V, 0);
       hipMemcpy(pe.data(), b->entries, pe.size() * 8, hipMemcpyDeviceToHost);
       for (uint32_t p = 0; p < b->num_passes; ++p)
         for (uint32_t v = 0; v < b->V; ++v) ent[v] += pe[size_t(p) * b->V + v];
       for (uint32_t v = 0; v < b->V; ++v) {
+        // Per tile: W = entries examined over the passes; F = the saturated
+        // prefix, counted from the start of the first pass to the furthest last
+        // hit (the walks of the passes before the last one with a hit, plus that
+        // pass's furthest hit).
         uint64_t f = 0, w = 0;
         for (uint32_t t = 0; t < b->views[v].num_tiles; ++t) {
-          f += tl[b->views[v].tile_base + t];
-          w += tw[b->views[v].tile_base + t];
+          const size_t g = b->views[v].tile_base + t;
+          uint64_t before = 0, tile_f = 0;
+          for (uint32_t p = 0; p < b->num_passes; ++p) {
+            if (tl[p * nt + g]) tile_f = before + tl[p * nt + g];
+            before += tw[p * nt + g];
+          }
+          f += tile_f;
+          w += before;
         }
         counters[4 * v] = ent[v];
         counters[4 * v + 1] = f;
@@ -511,8 +565,7 @@ int gsr_batch_profile(gsr_batch* b, const float* gs, const float* sh, const floa
     }
   }
   for (auto& ev : prof.ev) hipEventDestroy(ev);
-  dfree(prof.tile_last);
-  dfree(prof.tile_walk);
+  dfree(prof.last);
   dfree(prof.touched);
   if (e != hipSuccess) return ctx_fail(b->ctx, GSR_E_HIP, "profiled step failed");
   return GSR_OK;

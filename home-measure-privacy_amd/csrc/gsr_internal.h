@@ -34,11 +34,20 @@ struct ViewDesc {
   uint64_t splat_base;  // view * n
 };
 
-// One unit of blend work: a 16x16 tile of one view.
+// One unit of blend work: bands [first, last] of a 16x16 tile of one view, band
+// b being tile rows 4b .. 4b + 3 (a whole tile is bands 0-3).  view packs the
+// view index (bits 0-23), the first band (24-25) and the last band (26-27).
 struct TileWork {
   uint32_t view;
   uint32_t xy;  // tx | ty << 16
 };
+
+__host__ __device__ inline TileWork make_work(uint32_t view, uint32_t tx, uint32_t ty, uint32_t first, uint32_t last) {
+  return TileWork{view | (first << 24) | (last << 26), tx | (ty << 16)};
+}
+__host__ __device__ inline uint32_t work_view(const TileWork& w) { return w.view & 0xFFFFFFu; }
+__host__ __device__ inline uint32_t work_first(const TileWork& w) { return (w.view >> 24) & 3u; }
+__host__ __device__ inline uint32_t work_last(const TileWork& w) { return (w.view >> 26) & 3u; }
 
 // ---- launch helpers (host side), implemented per translation unit -------------
 
@@ -116,15 +125,43 @@ struct RasterArgs {
   uint64_t grads_view_stride;  // elements between views' gradient records (0: shared)
   uint32_t* queue;       // persistent mode: work counter (zeroed before launch)
   uint32_t tile_batch;   // tiles claimed per queue pop
+  // Band splitting (batched step), on when split != null: a tile with split[g]
+  // set is blended by four waves, one band each.  Items [0, heavy_count[0]) are
+  // those band items, then come the num_work tiles of `work` in order, the split
+  // ones returning at once.  Both are written by launch_split_list.
+  const uint8_t* split;          // [num_work]
+  const TileWork* heavy;         // [4 num_work]
+  const uint32_t* heavy_count;
   // Forward over passes: pass blends segment `pass`, resuming from tstate (per
-  // pixel transmittance) and image; done[g] is set once all of tile g's pixels
-  // have T <= 1e-4, and later passes skip the tile.
+  // pixel transmittance) and image, and skips tiles flagged done (saturated by
+  // earlier passes).  band_open[4 g + b] receives whether band b of tile g still
+  // has a pixel with T > 1e-4 after this pass.
   int pass;
   float* tstate;         // null for a single pass
-  uint8_t* done;         // null for a single pass
-  uint32_t* tile_last;   // forward, optional: per global tile, furthest last hit - start (over passes)
-  uint32_t* tile_walk;   // forward, optional: per global tile, entries examined (over passes)
+  const uint8_t* done;   // null for a single pass
+  uint8_t* band_open;    // null for a single pass
+  uint32_t* walk;        // forward, optional: per global tile, entries examined in this pass (max over items)
+  uint32_t* last;        // forward, optional: per global tile, furthest last hit - segment start (max over items)
 };
+// done[g] = (no band of tile g open) with assign != 0 (after the first pass, which
+// blends every tile), else done[g] |= (no band open).
+void launch_done_update(hipStream_t s, uint32_t total_tiles, const uint8_t* band_open, uint8_t* done, int assign);
+// Marks the tiles of a blend launch whose cost exceeds split_cost and lists
+// their band items (heavy_count zeroed before).  Cost of tile g: with walk, the
+// entries the forward walked summed over num_passes (walk[p][g], the backward's
+// estimate); otherwise the tile's entries in ranges (one forward pass).
+struct SplitArgs {
+  const TileWork* tiles;  // [total_tiles], whole-tile items
+  uint32_t total_tiles;
+  uint32_t split_cost;
+  const uint32_t* ranges;
+  const uint32_t* walk;
+  int num_passes;
+  uint8_t* split;
+  TileWork* heavy;
+  uint32_t* heavy_count;
+};
+void launch_split_list(hipStream_t s, const SplitArgs& a);
 void launch_forward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid);
 void launch_backward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid);
 // Resident blocks per CU of the persistent forward and backward kernels.

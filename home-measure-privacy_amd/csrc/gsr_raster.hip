@@ -36,13 +36,31 @@ struct Tile {
   const float* splats;
   uint64_t pixel_base;
   uint32_t gtile;       // global tile id (ranges index)
+  int first, last;      // bands of the tile this item covers
 };
 
-__device__ inline Tile load_tile(const RasterArgs& a, uint32_t w) {
-  TileWork tw = a.work[w];
-  ViewDesc vd = a.views[tw.view];
-  Tile t;
-  t.view = tw.view;
+// Items of a launch: the band items of split tiles, then one per tile.
+__device__ __forceinline__ uint32_t num_items(const RasterArgs& a) {
+  return (a.split ? a.heavy_count[0] : 0u) + a.num_work;
+}
+
+// The tile (and bands) of item w; false for the whole-tile items of split tiles.
+__device__ __forceinline__ bool load_tile(const RasterArgs& a, uint32_t w, Tile& t) {
+  const uint32_t nh = a.split ? a.heavy_count[0] : 0u;
+  TileWork tw;
+  if (w < nh) {
+    tw = a.heavy[w];
+  } else {
+    const uint32_t g = w - nh;
+    if (a.split && a.split[g]) return false;
+    tw = a.work[g];
+  }
+  const uint32_t first = work_first(tw), last = work_last(tw);
+  const uint32_t view = work_view(tw);
+  ViewDesc vd = a.views[view];
+  t.view = view;
+  t.first = int(first);
+  t.last = int(last);
   uint32_t tx = tw.xy & 0xFFFFu, ty = tw.xy >> 16;
   t.x0 = tx * kTile;
   t.y0 = ty * kTile;
@@ -53,13 +71,13 @@ __device__ inline Tile load_tile(const RasterArgs& a, uint32_t w) {
   t.order = a.order;
   t.splats = a.splats + vd.splat_base * 12;
   t.pixel_base = vd.pixel_base;
-  return t;
+  return true;
 }
 
 // Points t at its entries of pass p.  A batched step whose entries overflowed the
 // order storage is rerun; its ranges then point past the storage, so such
 // segments read nothing.
-__device__ inline void tile_segment(const RasterArgs& a, int p, Tile& t) {
+__device__ __forceinline__ void tile_segment(const RasterArgs& a, int p, Tile& t) {
   const uint64_t eb = a.ebase[p] ? a.ebase[p][t.view] : 0;
   t.start = a.ranges[p][size_t(t.gtile) * 2];
   t.end = a.ranges[p][size_t(t.gtile) * 2 + 1];
@@ -176,20 +194,37 @@ __device__ inline bool box_culled(const Box& b, float u, float v, float ex, floa
   return (b.x1 - u) < -ex || (b.x0 - u) > ex || (b.y1 - v) < -ey || (b.y0 - v) > ey;
 }
 
-// Lane rows (lane >> 4) + 4 i form band i: tile rows 4 i .. 4 i + 3.  True when
-// the entry's hit box misses the band's pixel centres [ylo, ylo + 3].
-__device__ inline bool band_culled(float ylo, float v, float ey) {
-  return (ylo + 3.0f - v) < -ey || (ylo - v) > ey;
+// Lane rows (lane >> 4) + 4 i form band i: tile rows 4 i .. 4 i + 3, pixel
+// centres band0 + 4 i .. band0 + 4 i + 3.  The bands an entry's hit box (|dy| <=
+// ey around v) can reach are [lo, hi]: band i needs 4 i + 3 >= v - ey - band0
+// and 4 i <= v + ey - band0.  The bounds are widened by a margin covering the
+// float32 rounding of these sums, so a band outside them holds no pixel whose
+// own offset py - v is within ey.  Returned in scalar registers (hi < lo: none).
+struct Bands {
+  int lo, hi;
+};
+
+__device__ inline Bands band_range(float band0, float v, float ey) {
+  const float margin = 1e-3f + 1e-6f * (fabsf(v) + fabsf(ey) + fabsf(band0));
+  const float flo = ceilf((((v - ey) - band0) - 3.0f - margin) * 0.25f);
+  const float fhi = floorf((((v + ey) - band0) + margin) * 0.25f);
+  // clamp to [-1, 4] before the conversion (NaN -> empty)
+  const int lo = int(fminf(fmaxf(flo, -1.0f), 4.0f));
+  const int hi = int(fminf(fmaxf(fhi, -1.0f), 4.0f));
+  const bool ok = flo == flo && fhi == fhi;
+  return Bands{__builtin_amdgcn_readfirstlane(ok ? lo : 4), __builtin_amdgcn_readfirstlane(ok ? hi : -1)};
 }
 
 // ---- forward -------------------------------------------------------------------
 
 // kLast: also track each pixel's last hit (the optional state output and the
 // profiling counters need it; the batched training step does not).
+// Inlined into the kernels, so the kernel arguments are read in place.
 template <bool kLast>
-__device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) {
+__device__ __forceinline__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) {
   const uint32_t lane = __lane_id();
-  Tile t = load_tile(a, w);
+  Tile t;
+  if (!load_tile(a, w, t)) return;
   const bool resume = a.pass > 0;
   if (resume && a.done[t.gtile]) return;  // saturated by an earlier pass
   tile_segment(a, a.pass, t);
@@ -201,8 +236,8 @@ __device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) 
 #pragma unroll
   for (int i = 0; i < kPix; ++i) {
     uint32_t y = t.y0 + (lane >> 4) + 4 * i;
-    bool inside = x < t.width && y < t.height;
-    T[i] = inside ? 1.0f : 0.0f;  // outside pixels count as done
+    bool inside = x < t.width && y < t.height && i >= t.first && i <= t.last;
+    T[i] = inside ? 1.0f : 0.0f;  // pixels outside the image or the item count as done
     cr[i] = cg[i] = cb[i] = 0.0f;
     last[i] = t.start;
     if (resume && inside) {
@@ -224,23 +259,32 @@ __device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) 
     if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
     const uint32_t nk = min(64u, t.end - k0);
     Box box = active_box(px, py0, T);
+    // The next entry's record is read from LDS while this one blends.
+    float4 c0 = lds[0][0], c1 = lds[0][1], c2 = lds[0][2];
     for (uint32_t j = 0; j < nk; ++j) {
       if ((j & 15) == 0 && j != 0) {
         if (!any_active(T)) break;
         if ((j & 31) == 0) box = active_box(px, py0, T);  // a stale box is a superset: still exact
       }
       walked = k0 + j + 1;
-      const float4 r0 = lds[j][0], r1 = lds[j][1], r2 = lds[j][2];
+      const float4 r0 = c0, r1 = c1, r2 = c2;
+      const uint32_t jn = j + 1 < nk ? j + 1 : j;
+      c0 = lds[jn][0];
+      c1 = lds[jn][1];
+      c2 = lds[jn][2];
       const float u = r0.x, v = r0.y, op = r1.y, rad = r1.z, ex = r1.w, ey = r2.w;
       if (box_culled(box, u, v, ex, ey)) continue;
       const Conic2 q = conic2(r0.z, r0.w, r1.x);
       const Row row = row_terms(q, px - u, rad);
+      const Bands bands = band_range(band0, v, ey);
+      const int blo = max(bands.lo, t.first), bhi = min(bands.hi, t.last);
 #pragma unroll
       for (int i = 0; i < kPix; ++i) {
-        // A band outside the hit box, a miss or a done pixel blends with weight 0:
-        // colour and T unchanged.
-        if (band_culled(band0 + 4.0f * i, v, ey)) continue;
+        // A band outside the hit box or the item, a miss or a done pixel blends
+        // with weight 0: colour and T unchanged.
+        if (i < blo || i > bhi) continue;
         const float alpha = alpha_at((py0 + 4.0f * i) - v, q, row, op, rad, T[i]);
+        if (__ballot(alpha != 0.0f) == 0) continue;
         const float wgt = alpha * T[i];
         cr[i] = fmaf(r2.x, wgt, cr[i]);
         cg[i] = fmaf(r2.y, wgt, cg[i]);
@@ -250,28 +294,28 @@ __device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) 
       }
     }
   }
-  if (kLast && a.tile_last) {
-    // Profiling over passes: the tile's saturated prefix (furthest last hit,
-    // counted from the start of its first pass) and the entries examined.
+  // Entries examined in this pass (the backward's work estimate and a profiling
+  // counter) and, when profiling, the furthest last hit: maxima over the items
+  // of the tile.
+  if (a.walk && lane == 0) atomicMax(a.walk + t.gtile, walked - t.start);
+  if (kLast && a.last) {
     uint32_t m = t.start;
 #pragma unroll
     for (int i = 0; i < kPix; ++i) m = max(m, last[i]);
     for (int off = 32; off > 0; off >>= 1) m = max(m, uint32_t(__shfl_xor(int(m), off)));
-    if (lane == 0) {
-      const uint32_t before = resume ? a.tile_walk[t.gtile] : 0u;
-      if (!resume) a.tile_last[t.gtile] = 0;
-      if (m > t.start) a.tile_last[t.gtile] = before + (m - t.start);
-      a.tile_walk[t.gtile] = before + (walked - t.start);
-    }
+    if (lane == 0 && m > t.start) atomicMax(a.last + t.gtile, m - t.start);
   }
-  if (a.done) {
-    const bool open = any_active(T);
-    if (lane == 0) a.done[t.gtile] = open ? 0 : 1;
+  if (a.band_open) {
+#pragma unroll
+    for (int i = 0; i < kPix; ++i) {
+      const bool open = __ballot(T[i] > kTStop) != 0;
+      if (lane == 0 && i >= t.first && i <= t.last) a.band_open[size_t(t.gtile) * 4 + i] = open ? 1 : 0;
+    }
   }
 #pragma unroll
   for (int i = 0; i < kPix; ++i) {
     uint32_t y = t.y0 + (lane >> 4) + 4 * i;
-    if (x >= t.width || y >= t.height) continue;
+    if (x >= t.width || y >= t.height || i < t.first || i > t.last) continue;
     size_t pix = t.pixel_base + size_t(y) * t.width + x;
     a.image[pix * 3] = cr[i];
     a.image[pix * 3 + 1] = cg[i];
@@ -284,8 +328,8 @@ __device__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) 
   }
 }
 
-__device__ inline void forward_any(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) {
-  if (a.state || a.tile_last) forward_tile<true>(a, w, lds);
+__device__ __forceinline__ void forward_any(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) {
+  if (a.state || a.last) forward_tile<true>(a, w, lds);
   else forward_tile<false>(a, w, lds);
 }
 
@@ -302,10 +346,11 @@ __device__ inline uint32_t pop_work(uint32_t* queue, uint32_t n) {
 
 __global__ __launch_bounds__(64) void forward_persistent_kernel(RasterArgs a) {
   __shared__ float4 lds[64][3];
+  const uint32_t items = num_items(a);
   for (;;) {
     uint32_t w0 = pop_work(a.queue, a.tile_batch);
-    if (w0 >= a.num_work) break;
-    uint32_t w1 = min(a.num_work, w0 + a.tile_batch);
+    if (w0 >= items) break;
+    uint32_t w1 = min(items, w0 + a.tile_batch);
     for (uint32_t w = w0; w < w1; ++w) forward_any(a, w, lds);
   }
 }
@@ -362,9 +407,10 @@ __device__ inline int reduce9_slot(uint32_t lane) {
   return lane == 2 ? 8 : -1;
 }
 
-__device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3], uint32_t* gi_lds) {
+__device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3], uint32_t* gi_lds) {
   const uint32_t lane = __lane_id();
-  Tile t = load_tile(a, w);
+  Tile t;
+  if (!load_tile(a, w, t)) return;
   const uint32_t x = t.x0 + (lane & 15);
   const float px = float(x) + 0.5f;
   const int slot = reduce9_slot(lane);
@@ -376,7 +422,7 @@ __device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3],
 #pragma unroll
   for (int i = 0; i < kPix; ++i) {
     uint32_t y = t.y0 + (lane >> 4) + 4 * i;
-    bool inside = x < t.width && y < t.height;
+    bool inside = x < t.width && y < t.height && i >= t.first && i <= t.last;
     T[i] = inside ? 1.0f : 0.0f;
     ar[i] = ag[i] = ab[i] = fr[i] = fg[i] = fb[i] = dr[i] = dg[i] = db[i] = 0.0f;
     if (inside) {
@@ -421,14 +467,18 @@ __device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3],
 #pragma unroll
       for (int m = 0; m < 9; ++m) g[m] = 0.0f;
       bool hit = false;
+      const Bands bands = band_range(band0, v, r2.w);
+      const int blo = max(bands.lo, t.first), bhi = min(bands.hi, t.last);
 #pragma unroll
       for (int i = 0; i < kPix; ++i) {
-        if (band_culled(band0 + 4.0f * i, v, r2.w)) continue;
+        if (i < blo || i > bhi) continue;
         const float dy = (py0 + 4.0f * i) - v;
         // Same alpha, transmittance and accumulated colour as forward_tile, bit for bit.
         const float alpha = alpha_at(dy, q, row, op, rad, T[i]);
         const bool h = alpha != 0.0f;
-        hit |= h;
+        // No pixel of the band hit: T, colour and gradients stay as they are.
+        if (__ballot(h) == 0) continue;
+        hit = true;
         const float Tk = T[i];
         const float wgt = alpha * Tk;
         const float oma = 1.0f - alpha;
@@ -455,7 +505,7 @@ __device__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3],
         g[7] = fmaf(dPdy, ndx, g[7]);
         g[8] = fmaf(dPdy, -0.5f * dy, g[8]);
       }
-      if (__ballot(hit) == 0) continue;
+      if (!hit) continue;  // wave-uniform: set only for bands with a hit
       const float val = reduce9(g, lane);
       if (slot >= 0 && val != 0.0f) unsafeAtomicAdd(grads + size_t(gi_lds[j]) * 16 + slot, val);
     }
@@ -472,10 +522,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ba
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void backward_persistent_kernel(RasterArgs a) {
   __shared__ float4 lds[64][3];
   __shared__ uint32_t gi_lds[64];
+  const uint32_t items = num_items(a);
   for (;;) {
     uint32_t w0 = pop_work(a.queue, a.tile_batch);
-    if (w0 >= a.num_work) break;
-    uint32_t w1 = min(a.num_work, w0 + a.tile_batch);
+    if (w0 >= items) break;
+    uint32_t w1 = min(items, w0 + a.tile_batch);
     for (uint32_t w = w0; w < w1; ++w) backward_tile(a, w, lds, gi_lds);
   }
 }
@@ -587,10 +638,48 @@ __global__ __launch_bounds__(256) void fill_work_kernel(const ViewDesc* __restri
   while (v + 1 < num_views && views[v + 1].tile_base <= g) ++v;
   const ViewDesc vd = views[v];
   const uint32_t t = g - vd.tile_base;
-  TileWork w;
-  w.view = v;
-  w.xy = (t % vd.tiles_x) | ((t / vd.tiles_x) << 16);
-  work[g] = w;
+  work[g] = make_work(v, t % vd.tiles_x, t / vd.tiles_x, 0, 3);
+}
+
+// One thread per tile; the band items of split tiles are appended with one
+// atomic per wave.
+__global__ __launch_bounds__(256) void split_list_kernel(SplitArgs a) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  bool split = false;
+  if (g < a.total_tiles) {
+    uint32_t cost = 0;
+    if (a.walk) {
+      for (int p = 0; p < a.num_passes; ++p) cost += a.walk[size_t(p) * a.total_tiles + g];
+    } else {
+      cost = a.ranges[size_t(g) * 2 + 1] - a.ranges[size_t(g) * 2];
+    }
+    split = cost > a.split_cost;
+    a.split[g] = split ? 1 : 0;
+  }
+  const uint64_t m = __ballot(split);
+  if (m == 0) return;
+  const uint32_t lane = __lane_id();
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(a.heavy_count, 4u * uint32_t(__popcll(m)));
+  base = __shfl(base, 0);
+  if (!split) return;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const uint32_t at = base + 4u * uint32_t(__popcll(m & below));
+  const TileWork w = a.tiles[g];
+  const uint32_t v = work_view(w), tx = w.xy & 0xFFFFu, ty = w.xy >> 16;
+  for (uint32_t b = 0; b < 4; ++b) a.heavy[at + b] = make_work(v, tx, ty, b, b);
+}
+
+// After the first pass (which blends every tile) done is assigned; after later
+// passes it only gains tiles (tiles a pass skipped keep their flags).
+__global__ __launch_bounds__(256) void done_update_kernel(uint32_t total_tiles,
+                                                          const uint32_t* __restrict__ band_open,
+                                                          uint8_t* __restrict__ done, int assign) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total_tiles) return;
+  const bool closed = band_open[g] == 0;
+  if (assign) done[g] = closed ? 1 : 0;
+  else if (closed) done[g] = 1;
 }
 
 }  // namespace
@@ -600,6 +689,17 @@ void launch_fill_work(hipStream_t s, const ViewDesc* views, uint32_t num_views, 
   if (total_tiles == 0) return;
   hipLaunchKernelGGL(fill_work_kernel, dim3((total_tiles + 255) / 256), dim3(256), 0, s, views, num_views,
                      total_tiles, work);
+}
+
+void launch_split_list(hipStream_t s, const SplitArgs& a) {
+  if (a.total_tiles == 0) return;
+  hipLaunchKernelGGL(split_list_kernel, dim3((a.total_tiles + 255) / 256), dim3(256), 0, s, a);
+}
+
+void launch_done_update(hipStream_t s, uint32_t total_tiles, const uint8_t* band_open, uint8_t* done, int assign) {
+  if (total_tiles == 0) return;
+  hipLaunchKernelGGL(done_update_kernel, dim3((total_tiles + 255) / 256), dim3(256), 0, s, total_tiles,
+                     reinterpret_cast<const uint32_t*>(band_open), done, assign);
 }
 
 void launch_forward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid) {

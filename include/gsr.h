@@ -189,7 +189,7 @@ int gsr_batch_configure(gsr_batch* batch, uint32_t tile_batch, uint32_t chunk);
  * entry.  growth is ignored for passes == 1. */
 int gsr_batch_set_passes(gsr_batch* batch, uint32_t passes, uint32_t growth);
 /* Runs one step without the graph, timing each stage with HIP events on the
- * context's stream.  stage_ms[GSR_STAGES] receives milliseconds for: project,
+ * context's stream, then a second step that collects the counters.  stage_ms[GSR_STAGES] receives milliseconds for: project,
  * depth sort, bin count, bin emit, forward, loss gradient, backward, chain.
  * counters[4 * num_views] receives per view: entries E, saturated-prefix entries
  * F (sum over tiles of the furthest pixel's last hit minus the tile start),
