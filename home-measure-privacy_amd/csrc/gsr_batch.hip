@@ -273,9 +273,18 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
     a.split = split_cost ? b->split : nullptr;
     a.heavy = b->heavy;
     a.heavy_count = b->flags + 16 + p;
-    if (split_cost)
-      launch_split_list(s, SplitArgs{b->d_work, b->total_tiles, split_cost, a.ranges[p], nullptr, 0, b->split,
-                                     b->heavy, b->flags + 16 + p});
+    if (split_cost) {
+      SplitArgs sa{};
+      sa.tiles = b->d_work;
+      sa.total_tiles = b->total_tiles;
+      sa.split_cost = split_cost;
+      sa.ranges[0] = a.ranges[p];
+      sa.num_passes = 1;
+      sa.split = b->split;
+      sa.heavy = b->heavy;
+      sa.heavy_count = b->flags + 16 + p;
+      launch_split_list(s, sa);
+    }
     a.walk = b->walk + size_t(p) * b->total_tiles;
     a.last = prof && prof->last ? prof->last + size_t(p) * b->total_tiles : nullptr;
     launch_forward(s, a, 1, b->grid_fwd);
@@ -305,9 +314,19 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   mark();
   a.split = b->split_backward ? b->split : nullptr;
   a.heavy_count = b->flags + 16 + kMaxPasses;
-  if (b->split_backward)
-    launch_split_list(s, SplitArgs{b->d_work, b->total_tiles, b->split_backward, nullptr, b->walk,
-                                   int(b->num_passes), b->split, b->heavy, b->flags + 16 + kMaxPasses});
+  if (b->split_backward) {
+    SplitArgs sa{};
+    sa.tiles = b->d_work;
+    sa.total_tiles = b->total_tiles;
+    sa.split_cost = b->split_backward;
+    for (uint32_t p = 0; p < b->num_passes; ++p) sa.ranges[p] = a.ranges[p];
+    sa.num_passes = int(b->num_passes);
+    sa.walk = b->walk;
+    sa.split = b->split;
+    sa.heavy = b->heavy;
+    sa.heavy_count = b->flags + 16 + kMaxPasses;
+    launch_split_list(s, sa);
+  }
   a.queue = b->flags + 1 + kMaxPasses;
   launch_backward(s, a, 1, b->grid_bwd);
   mark();

@@ -147,16 +147,16 @@ struct RasterArgs {
 // blends every tile), else done[g] |= (no band open).
 void launch_done_update(hipStream_t s, uint32_t total_tiles, const uint8_t* band_open, uint8_t* done, int assign);
 // Marks the tiles of a blend launch whose cost exceeds split_cost and lists
-// their band items (heavy_count zeroed before).  Cost of tile g: with walk, the
-// entries the forward walked summed over num_passes (walk[p][g], the backward's
-// estimate); otherwise the tile's entries in ranges (one forward pass).
+// their band items (heavy_count zeroed before).  Cost of tile g: with walk (the
+// backward), the entries the forward examined summed over num_passes
+// (walk[p][g]); otherwise its entries over ranges[0 .. num_passes).
 struct SplitArgs {
   const TileWork* tiles;  // [total_tiles], whole-tile items
   uint32_t total_tiles;
   uint32_t split_cost;
-  const uint32_t* ranges;
-  const uint32_t* walk;
+  const uint32_t* ranges[kMaxPasses];
   int num_passes;
+  const uint32_t* walk;        // optional, [num_passes][total_tiles]
   uint8_t* split;
   TileWork* heavy;
   uint32_t* heavy_count;

@@ -194,6 +194,13 @@ __device__ inline bool box_culled(const Box& b, float u, float v, float ex, floa
   return (b.x1 - u) < -ex || (b.x0 - u) > ex || (b.y1 - v) < -ey || (b.y0 - v) > ey;
 }
 
+// Lane rows (lane >> 4) + 4 i form band i: tile rows 4 i .. 4 i + 3.  True when
+// the entry's hit box misses the band's pixel centres [ylo, ylo + 3] (the
+// subtractions are the pixels' own py - v, monotonic in py).
+__device__ inline bool band_culled(float ylo, float v, float ey) {
+  return (ylo + 3.0f - v) < -ey || (ylo - v) > ey;
+}
+
 // Lane rows (lane >> 4) + 4 i form band i: tile rows 4 i .. 4 i + 3, pixel
 // centres band0 + 4 i .. band0 + 4 i + 3.  The bands an entry's hit box (|dy| <=
 // ey around v) can reach are [lo, hi]: band i needs 4 i + 3 >= v - ey - band0
@@ -250,41 +257,37 @@ __device__ __forceinline__ void forward_tile(const RasterArgs& a, uint32_t w, fl
   }
   const float band0 = float(t.y0) + 0.5f;
   Batch nb = fetch_batch(t, t.start, lane);
-  uint32_t walked = t.start;
+  uint32_t walked = t.start;  // exact, kLast only
+  uint32_t stop = t.end;      // start of the batch at which the tile had no active pixel
   for (uint32_t k0 = t.start; k0 < t.end; k0 += 64) {
-    if (!any_active(T)) break;
+    if (!any_active(T)) {
+      stop = k0;
+      break;
+    }
     __syncthreads();
     stage(lds, lane, nb);
     __syncthreads();
     if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
     const uint32_t nk = min(64u, t.end - k0);
     Box box = active_box(px, py0, T);
-    // The next entry's record is read from LDS while this one blends.
-    float4 c0 = lds[0][0], c1 = lds[0][1], c2 = lds[0][2];
     for (uint32_t j = 0; j < nk; ++j) {
       if ((j & 15) == 0 && j != 0) {
         if (!any_active(T)) break;
         if ((j & 31) == 0) box = active_box(px, py0, T);  // a stale box is a superset: still exact
       }
-      walked = k0 + j + 1;
-      const float4 r0 = c0, r1 = c1, r2 = c2;
-      const uint32_t jn = j + 1 < nk ? j + 1 : j;
-      c0 = lds[jn][0];
-      c1 = lds[jn][1];
-      c2 = lds[jn][2];
+      if (kLast) walked = k0 + j + 1;
+      const float4 r0 = lds[j][0], r1 = lds[j][1], r2 = lds[j][2];
       const float u = r0.x, v = r0.y, op = r1.y, rad = r1.z, ex = r1.w, ey = r2.w;
       if (box_culled(box, u, v, ex, ey)) continue;
       const Conic2 q = conic2(r0.z, r0.w, r1.x);
       const Row row = row_terms(q, px - u, rad);
-      const Bands bands = band_range(band0, v, ey);
-      const int blo = max(bands.lo, t.first), bhi = min(bands.hi, t.last);
 #pragma unroll
       for (int i = 0; i < kPix; ++i) {
         // A band outside the hit box or the item, a miss or a done pixel blends
-        // with weight 0: colour and T unchanged.
-        if (i < blo || i > bhi) continue;
+        // with weight 0: colour and T unchanged.  (State updated in place per
+        // band: this form compiles without copies of the pixel state.)
+        if (i < t.first || i > t.last || band_culled(band0 + 4.0f * i, v, ey)) continue;
         const float alpha = alpha_at((py0 + 4.0f * i) - v, q, row, op, rad, T[i]);
-        if (__ballot(alpha != 0.0f) == 0) continue;
         const float wgt = alpha * T[i];
         cr[i] = fmaf(r2.x, wgt, cr[i]);
         cg[i] = fmaf(r2.y, wgt, cg[i]);
@@ -294,10 +297,18 @@ __device__ __forceinline__ void forward_tile(const RasterArgs& a, uint32_t w, fl
       }
     }
   }
-  // Entries examined in this pass (the backward's work estimate and a profiling
-  // counter) and, when profiling, the furthest last hit: maxima over the items
-  // of the tile.
-  if (a.walk && lane == 0) atomicMax(a.walk + t.gtile, walked - t.start);
+  // Entries examined in this pass (the backward's work estimate; exact when
+  // profiling, else to the batch) and, when profiling, the furthest last hit:
+  // maxima over the items of the tile.  (Exact counting stays out of the
+  // regular blend, which then runs with fewer registers and more waves.)  Then
+  // which bands stay open: a whole-tile item is its tile's only writer; band
+  // items of a split tile write one byte per band.
+  const bool whole = t.first == 0 && t.last == kPix - 1;
+  if (a.walk && lane == 0) {
+    const uint32_t n = (kLast ? walked : stop) - t.start;
+    if (whole) a.walk[t.gtile] = n;
+    else atomicMax(a.walk + t.gtile, n);
+  }
   if (kLast && a.last) {
     uint32_t m = t.start;
 #pragma unroll
@@ -306,10 +317,15 @@ __device__ __forceinline__ void forward_tile(const RasterArgs& a, uint32_t w, fl
     if (lane == 0 && m > t.start) atomicMax(a.last + t.gtile, m - t.start);
   }
   if (a.band_open) {
+    if (whole) {
+      const bool open = any_active(T);
+      if (lane == 0) reinterpret_cast<uint32_t*>(a.band_open)[t.gtile] = open ? 0x01010101u : 0u;
+    } else {
 #pragma unroll
-    for (int i = 0; i < kPix; ++i) {
-      const bool open = __ballot(T[i] > kTStop) != 0;
-      if (lane == 0 && i >= t.first && i <= t.last) a.band_open[size_t(t.gtile) * 4 + i] = open ? 1 : 0;
+      for (int i = 0; i < kPix; ++i) {
+        const bool open = __ballot(T[i] > kTStop) != 0;
+        if (lane == 0 && i >= t.first && i <= t.last) a.band_open[size_t(t.gtile) * 4 + i] = open ? 1 : 0;
+      }
     }
   }
 #pragma unroll
@@ -328,14 +344,12 @@ __device__ __forceinline__ void forward_tile(const RasterArgs& a, uint32_t w, fl
   }
 }
 
-__device__ __forceinline__ void forward_any(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) {
-  if (a.state || a.last) forward_tile<true>(a, w, lds);
-  else forward_tile<false>(a, w, lds);
-}
-
+// kLast (the optional state output and the profiling counters) is a kernel of
+// its own, so the regular blend keeps its smaller register budget.
+template <bool kLast>
 __global__ __launch_bounds__(64) void forward_kernel(RasterArgs a) {
   __shared__ float4 lds[64][3];
-  forward_any(a, blockIdx.x, lds);
+  forward_tile<kLast>(a, blockIdx.x, lds);
 }
 
 __device__ inline uint32_t pop_work(uint32_t* queue, uint32_t n) {
@@ -344,6 +358,7 @@ __device__ inline uint32_t pop_work(uint32_t* queue, uint32_t n) {
   return __builtin_amdgcn_readfirstlane(w0);
 }
 
+template <bool kLast>
 __global__ __launch_bounds__(64) void forward_persistent_kernel(RasterArgs a) {
   __shared__ float4 lds[64][3];
   const uint32_t items = num_items(a);
@@ -351,7 +366,7 @@ __global__ __launch_bounds__(64) void forward_persistent_kernel(RasterArgs a) {
     uint32_t w0 = pop_work(a.queue, a.tile_batch);
     if (w0 >= items) break;
     uint32_t w1 = min(items, w0 + a.tile_batch);
-    for (uint32_t w = w0; w < w1; ++w) forward_any(a, w, lds);
+    for (uint32_t w = w0; w < w1; ++w) forward_tile<kLast>(a, w, lds);
   }
 }
 
@@ -651,7 +666,7 @@ __global__ __launch_bounds__(256) void split_list_kernel(SplitArgs a) {
     if (a.walk) {
       for (int p = 0; p < a.num_passes; ++p) cost += a.walk[size_t(p) * a.total_tiles + g];
     } else {
-      cost = a.ranges[size_t(g) * 2 + 1] - a.ranges[size_t(g) * 2];
+      for (int p = 0; p < a.num_passes; ++p) cost += a.ranges[p][size_t(g) * 2 + 1] - a.ranges[p][size_t(g) * 2];
     }
     split = cost > a.split_cost;
     a.split[g] = split ? 1 : 0;
@@ -704,10 +719,14 @@ void launch_done_update(hipStream_t s, uint32_t total_tiles, const uint8_t* band
 
 void launch_forward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid) {
   if (a.num_work == 0) return;
-  if (persistent)
-    hipLaunchKernelGGL(forward_persistent_kernel, dim3(grid), dim3(64), 0, s, a);
-  else
-    hipLaunchKernelGGL(forward_kernel, dim3(a.num_work), dim3(64), 0, s, a);
+  const bool track = a.state || a.last;
+  if (persistent) {
+    if (track) hipLaunchKernelGGL(forward_persistent_kernel<true>, dim3(grid), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(forward_persistent_kernel<false>, dim3(grid), dim3(64), 0, s, a);
+  } else {
+    if (track) hipLaunchKernelGGL(forward_kernel<true>, dim3(a.num_work), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(forward_kernel<false>, dim3(a.num_work), dim3(64), 0, s, a);
+  }
 }
 
 void launch_backward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid) {
@@ -720,7 +739,7 @@ void launch_backward(hipStream_t s, const RasterArgs& a, int persistent, uint32_
 
 void raster_occupancy(int* forward_per_cu, int* backward_per_cu) {
   int f = 0, b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&f, forward_persistent_kernel, 64, 0) != hipSuccess) f = 8;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&f, forward_persistent_kernel<false>, 64, 0) != hipSuccess) f = 8;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, backward_persistent_kernel, 64, 0) != hipSuccess) b = 8;
   *forward_per_cu = f;
   *backward_per_cu = b;
