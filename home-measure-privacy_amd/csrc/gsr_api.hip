@@ -344,8 +344,15 @@ int gsr_forward(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, const 
 
 int gsr_loss_grad(gsr_ctx* ctx, const gsr_camera* cam, const float* image, const float* target, float lambda,
                   float* loss_grad) {
+  return gsr_loss_grad_window(ctx, cam, image, target, lambda, GSR_SSIM_BOX7, loss_grad);
+}
+
+int gsr_loss_grad_window(gsr_ctx* ctx, const gsr_camera* cam, const float* image, const float* target,
+                         float lambda, int window, float* loss_grad) {
   if (!ctx) return GSR_E_INVALID;
   if (!image || !target || !loss_grad) return fail(ctx, GSR_E_INVALID, "NULL argument");
+  if (window != GSR_SSIM_BOX7 && window != GSR_SSIM_GAUSS11)
+    return fail(ctx, GSR_E_INVALID, "unknown SSIM window %d", window);
   if (!(lambda >= 0.0f && lambda <= 1.0f)) return fail(ctx, GSR_E_INVALID, "lambda %g not in [0, 1]", lambda);
   ViewDesc* dview;
   gsr_camera* dcam;
@@ -354,7 +361,7 @@ int gsr_loss_grad(gsr_ctx* ctx, const gsr_camera* cam, const float* image, const
   if (rc) return rc;
   float* coef = nullptr;
   if (lambda > 0.0f && (rc = pooled_t(ctx, "ssim_coef", size_t(host.width) * host.height * 9, &coef))) return rc;
-  launch_loss_grad(ctx->stream, dview, &host, 1, image, target, lambda, coef, loss_grad);
+  launch_loss_grad(ctx->stream, dview, &host, 1, image, target, lambda, coef, loss_grad, window);
   GSR_LAUNCHED(ctx);
   return GSR_OK;
 }

@@ -61,6 +61,7 @@ struct gsr_batch {
   float* tstate = nullptr;      // per pixel transmittance between forward passes
   uint8_t* done = nullptr;      // per tile: saturated
   uint8_t* band_open = nullptr; // per tile, 4 bytes: band still open after the last forward pass
+  int ssim_window = GSR_SSIM_BOX7;
   uint32_t* walk = nullptr;     // [kMaxPasses][total_tiles]: entries the forward examined per pass
   TileWork* heavy = nullptr;    // [4 total_tiles]: band items of the split tiles of a launch
   uint8_t* split = nullptr;     // [total_tiles]: tile split in the current launch
@@ -310,7 +311,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   a.walk = nullptr;
   a.last = nullptr;
   mark();
-  launch_loss_grad(s, b->d_views, b->views.data(), b->V, image, targets, lambda, b->coef, b->dl);
+  launch_loss_grad(s, b->d_views, b->views.data(), b->V, image, targets, lambda, b->coef, b->dl, b->ssim_window);
   mark();
   a.split = b->split_backward ? b->split : nullptr;
   a.heavy_count = b->flags + 16 + kMaxPasses;
@@ -488,6 +489,17 @@ int gsr_batch_set_passes(gsr_batch* b, uint32_t passes, uint32_t growth) {
   for (uint32_t p = 1; p < passes; ++p) div *= growth;
   if (div > b->n) return ctx_fail(b->ctx, GSR_E_INVALID, "too few Gaussians for the passes");
   if (passes != b->num_passes || growth != b->growth) set_passes(b, passes, growth);
+  return GSR_OK;
+}
+
+int gsr_batch_set_ssim_window(gsr_batch* b, int window) {
+  if (!b) return GSR_E_INVALID;
+  if (window != GSR_SSIM_BOX7 && window != GSR_SSIM_GAUSS11)
+    return ctx_fail(b->ctx, GSR_E_INVALID, "unknown SSIM window");
+  if (window != b->ssim_window) {
+    b->ssim_window = window;
+    drop_graph(b);
+  }
   return GSR_OK;
 }
 

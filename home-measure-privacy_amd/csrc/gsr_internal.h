@@ -166,8 +166,10 @@ void launch_forward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t
 void launch_backward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid);
 // Resident blocks per CU of the persistent forward and backward kernels.
 void raster_occupancy(int* forward_per_cu, int* backward_per_cu);
+// window 7: the reference's 7x7 box SSIM; 11: the 11x11 Gaussian (sigma 1.5).
 void launch_loss_grad(hipStream_t s, const ViewDesc* views, const ViewDesc* host_views, uint32_t num_views,
-                      const float* image, const float* target, float lambda, float* coef, float* grad);
+                      const float* image, const float* target, float lambda, float* coef, float* grad,
+                      int window = 7);
 // Device copy / zero fill of 4-byte aligned memory, bytes a multiple of 4.
 void launch_copy(hipStream_t s, void* dst, const void* src, size_t bytes);
 void launch_zero(hipStream_t s, void* p, size_t bytes);

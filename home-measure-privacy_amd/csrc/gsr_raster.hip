@@ -548,30 +548,60 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ba
 
 // ---- loss gradient (gpu.js:336-409) ---------------------------------------------
 
+// SSIM windows.  kWin == 7: the reference's 7x7 box, weights 1/49 (gpu.js:342-
+// 372).  kWin == 11: an 11x11 Gaussian window, sigma 1.5, w(i, j) = g(i) g(j)
+// with g normalised to sum 1 (the window of the usual 3DGS loss).  Windows are
+// the valid ones (wholly inside the image); the mean SSIM is over windows and
+// channels, as gpu.js:747 counts them.
+template <int kWin>
+__device__ inline void window_weights(float w[kWin]) {
+  if (kWin == 7) {
+#pragma unroll
+    for (int k = 0; k < kWin; ++k) w[k] = 1.0f / 7.0f;
+  } else {
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kWin; ++k) {
+      const float d = float(k - kWin / 2);
+      w[k] = expf(-d * d / (2.0f * 1.5f * 1.5f));
+      sum += w[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kWin; ++k) w[k] /= sum;
+  }
+}
+
+// SSIM_COEF (gpu.js:342-372) for window kWin: per valid window centre and
+// channel, the coefficients (a, b, d) with dSSIM_window/dx_p = w_p (a + 2 b x_p +
+// d y_p), from the window's weighted moments.
+template <int kWin>
 __global__ __launch_bounds__(256) void ssim_coef_kernel(const ViewDesc* __restrict__ views,
                                                         const float* __restrict__ image,
                                                         const float* __restrict__ target, float* __restrict__ coef) {
+  constexpr int kR = kWin / 2;
   const ViewDesc vd = views[blockIdx.y];
   const uint32_t P = vd.width * vd.height;
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
   const int W = int(vd.width), H = int(vd.height);
   const int x = int(p % vd.width), y = int(p / vd.width);
-  const bool inside = x >= 3 && y >= 3 && x + 3 < W && y + 3 < H;
+  const bool inside = x >= kR && y >= kR && x + kR < W && y + kR < H;
   const float* img = image + vd.pixel_base * 3;
   const float* tgt = target + vd.pixel_base * 3;
   float* out = coef + (vd.pixel_base + p) * 9;
+  float wt[kWin];
+  window_weights<kWin>(wt);
   for (int c = 0; c < 3; ++c) {
     float ca = 0.0f, cb = 0.0f, cd = 0.0f;
     if (inside) {
       float mx = 0.0f, my = 0.0f, sxx = 0.0f, syy = 0.0f, sxy = 0.0f;
-      for (int dy = 0; dy < 7; ++dy)
-        for (int dx = 0; dx < 7; ++dx) {
-          size_t q = (size_t(y + dy - 3) * W + x + dx - 3) * 3 + c;
+      for (int dy = 0; dy < kWin; ++dy)
+        for (int dx = 0; dx < kWin; ++dx) {
+          size_t q = (size_t(y + dy - kR) * W + x + dx - kR) * 3 + c;
+          const float w = wt[dy] * wt[dx];
           float xv = img[q], yv = tgt[q];
-          mx += xv; my += yv; sxx += xv * xv; syy += yv * yv; sxy += xv * yv;
+          mx += w * xv; my += w * yv; sxx += w * xv * xv; syy += w * yv * yv; sxy += w * xv * yv;
         }
-      mx /= 49.0f; my /= 49.0f; sxx /= 49.0f; syy /= 49.0f; sxy /= 49.0f;
       const float c1 = 0.0001f, c2 = 0.0009f;
       float a1 = 2.0f * mx * my + c1;
       float a2 = 2.0f * (sxy - mx * my) + c2;
@@ -590,11 +620,16 @@ __global__ __launch_bounds__(256) void ssim_coef_kernel(const ViewDesc* __restri
 
 __device__ inline float sign_f(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
 
+// LOSS_GRAD (gpu.js:383-409): dLoss/dpixel of (1 - lambda) L1 + lambda (1 - SSIM)
+// = (1 - lambda) sign(x - y) / (3P) - lambda / windows * sum over the windows
+// holding the pixel of w (a + 2 b x + d y).
+template <int kWin>
 __global__ __launch_bounds__(256) void loss_grad_kernel(const ViewDesc* __restrict__ views,
                                                         const float* __restrict__ image,
                                                         const float* __restrict__ target,
                                                         const float* __restrict__ coef, float lambda,
                                                         float* __restrict__ grad) {
+  constexpr int kR = kWin / 2;
   const ViewDesc vd = views[blockIdx.y];
   const uint32_t P = vd.width * vd.height;
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -602,21 +637,25 @@ __global__ __launch_bounds__(256) void loss_grad_kernel(const ViewDesc* __restri
   const int W = int(vd.width), H = int(vd.height);
   const int x0 = int(p % vd.width), y0 = int(p / vd.width);
   const float n = float(P * 3u);
-  const float windows = float(max(0, W - 6) * max(0, H - 6) * 3);
+  const float windows = float(max(0, W - 2 * kR) * max(0, H - 2 * kR) * 3);
   const size_t pix = vd.pixel_base + p;
+  float wt[kWin];
+  if (lambda > 0.0f) window_weights<kWin>(wt);
   for (int c = 0; c < 3; ++c) {
     float xv = image[pix * 3 + c], yv = target[pix * 3 + c];
     float g = (1.0f - lambda) * sign_f(xv - yv) / n;
     if (lambda > 0.0f && windows > 0.0f) {
       float sa = 0.0f, sb = 0.0f, sd = 0.0f;
-      for (int dy = -3; dy <= 3; ++dy)
-        for (int dx = -3; dx <= 3; ++dx) {
+      for (int dy = -kR; dy <= kR; ++dy)
+        for (int dx = -kR; dx <= kR; ++dx) {
           int cx = x0 + dx, cy = y0 + dy;
           if (cx < 0 || cy < 0 || cx >= W || cy >= H) continue;
           const float* k = coef + (vd.pixel_base + size_t(cy) * W + cx) * 9 + c * 3;
-          sa += k[0]; sb += k[1]; sd += k[2];
+          // the pixel sits at offset (-dx, -dy) of the window centred at (cx, cy)
+          const float w = wt[kR - dy] * wt[kR - dx];
+          sa += w * k[0]; sb += w * k[1]; sd += w * k[2];
         }
-      float dS = (sa + 2.0f * xv * sb + yv * sd) / 49.0f / windows;
+      float dS = (sa + 2.0f * xv * sb + yv * sd) / windows;
       g -= lambda * dS;
     }
     grad[pix * 3 + c] = g;
@@ -746,7 +785,8 @@ void raster_occupancy(int* forward_per_cu, int* backward_per_cu) {
 }
 
 void launch_loss_grad(hipStream_t s, const ViewDesc* views, const ViewDesc* host_views, uint32_t num_views,
-                      const float* image, const float* target, float lambda, float* coef, float* grad) {
+                      const float* image, const float* target, float lambda, float* coef, float* grad,
+                      int window) {
   uint32_t maxp = 0;
   for (uint32_t v = 0; v < num_views; ++v) {
     uint32_t p = host_views[v].width * host_views[v].height;
@@ -754,8 +794,13 @@ void launch_loss_grad(hipStream_t s, const ViewDesc* views, const ViewDesc* host
   }
   if (maxp == 0) return;
   dim3 grid((maxp + 255) / 256, num_views);
-  if (lambda > 0.0f) hipLaunchKernelGGL(ssim_coef_kernel, grid, dim3(256), 0, s, views, image, target, coef);
-  hipLaunchKernelGGL(loss_grad_kernel, grid, dim3(256), 0, s, views, image, target, coef, lambda, grad);
+  if (window == 11) {
+    if (lambda > 0.0f) hipLaunchKernelGGL(ssim_coef_kernel<11>, grid, dim3(256), 0, s, views, image, target, coef);
+    hipLaunchKernelGGL(loss_grad_kernel<11>, grid, dim3(256), 0, s, views, image, target, coef, lambda, grad);
+  } else {
+    if (lambda > 0.0f) hipLaunchKernelGGL(ssim_coef_kernel<7>, grid, dim3(256), 0, s, views, image, target, coef);
+    hipLaunchKernelGGL(loss_grad_kernel<7>, grid, dim3(256), 0, s, views, image, target, coef, lambda, grad);
+  }
 }
 
 void launch_copy(hipStream_t s, void* dst, const void* src, size_t bytes) {

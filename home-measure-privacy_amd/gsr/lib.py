@@ -24,8 +24,9 @@ EXPORTS = (
     "gsr_clear", "gsr_copy", "gsr_synchronize", "gsr_project", "gsr_bin_sort", "gsr_forward", "gsr_loss_grad", "gsr_backward",
     "gsr_shape", "gsr_adam", "gsr_adam_sh", "gsr_stats", "gsr_batch_create", "gsr_batch_destroy", "gsr_batch_set_cameras",
     "gsr_batch_step", "gsr_batch_entries", "gsr_batch_configure", "gsr_batch_autotune", "gsr_batch_profile",
-    "gsr_batch_set_passes",
+    "gsr_batch_set_passes", "gsr_loss_grad_window", "gsr_batch_set_ssim_window",
 )
+SSIM_BOX7, SSIM_GAUSS11 = 7, 11
 STAGES = ("project", "depth_sort", "bin_count", "bin_emit", "forward", "loss_grad", "backward", "chain")
 
 
@@ -90,6 +91,8 @@ def load():
         "gsr_batch_autotune": (i32, [P, P, P, P, f32, P, P, P, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "gsr_batch_profile": (i32, [P, P, P, P, f32, P, P, P, P, P]),
         "gsr_batch_set_passes": (i32, [P, u32, u32]),
+        "gsr_loss_grad_window": (i32, [P, ctypes.POINTER(Camera), P, P, f32, i32, P]),
+        "gsr_batch_set_ssim_window": (i32, [P, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -209,9 +212,13 @@ class Context:
         self._check(self.lib.gsr_forward(self.h, ctypes.byref(c), splats.p, order.p, ranges.p, image.p,
                                          state.p if state else None))
 
-    def loss_grad(self, cam, image: Buffer, target: Buffer, lam: float, out: Buffer):
+    def loss_grad(self, cam, image: Buffer, target: Buffer, lam: float, out: Buffer, window: int = SSIM_BOX7):
         c = camera(cam, 0)
-        self._check(self.lib.gsr_loss_grad(self.h, ctypes.byref(c), image.p, target.p, lam, out.p))
+        if window == SSIM_BOX7:
+            self._check(self.lib.gsr_loss_grad(self.h, ctypes.byref(c), image.p, target.p, lam, out.p))
+        else:
+            self._check(self.lib.gsr_loss_grad_window(self.h, ctypes.byref(c), image.p, target.p, lam, window,
+                                                      out.p))
 
     def backward(self, cam, splats: Buffer, order: Buffer, ranges: Buffer, image: Buffer, dl: Buffer, grads: Buffer):
         c = camera(cam, 0)
@@ -274,6 +281,9 @@ class Batch:
 
     def set_passes(self, passes: int, growth: int = 2):
         self.ctx._check(self.lib.gsr_batch_set_passes(self.h, passes, growth))
+
+    def set_ssim_window(self, window: int):
+        self.ctx._check(self.lib.gsr_batch_set_ssim_window(self.h, window))
 
     def configure(self, tile_batch: int = 0, chunk: int = 0):
         self.ctx._check(self.lib.gsr_batch_configure(self.h, tile_batch, chunk))

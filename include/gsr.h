@@ -119,6 +119,16 @@ int gsr_forward(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, const 
 int gsr_loss_grad(gsr_ctx* ctx, const gsr_camera* cam, const float* image, const float* target, float lambda,
                   float* loss_grad);
 
+/* SSIM windows of the loss gradient: the reference's 7x7 box (gpu.js:342-372),
+ * and an 11x11 Gaussian window with sigma 1.5 (extension; the window of the
+ * usual 3D Gaussian splatting loss).  Both use the valid windows (wholly inside
+ * the image), C1 = 1e-4, C2 = 9e-4, and the mean over windows and channels. */
+#define GSR_SSIM_BOX7 7
+#define GSR_SSIM_GAUSS11 11
+/* gsr_loss_grad with the SSIM window chosen (GSR_SSIM_BOX7 or GSR_SSIM_GAUSS11). */
+int gsr_loss_grad_window(gsr_ctx* ctx, const gsr_camera* cam, const float* image, const float* target,
+                         float lambda, int window, float* loss_grad);
+
 /* BACKWARD_SG (gpu.js:204-293) semantics, walked front to back with the
  * transmittance recomputed from the splats (no per-pixel state): adds the
  * colour, opacity-logit and screen (u, v, A, B, C) gradients into grads
@@ -180,6 +190,8 @@ int gsr_batch_entries(gsr_batch* batch, uint64_t* entries);
  * binning chunk (0 = keep).  gsr_batch_autotune times candidates on the given
  * inputs and keeps the fastest; it writes the choice to *tile_batch / *chunk. */
 int gsr_batch_configure(gsr_batch* batch, uint32_t tile_batch, uint32_t chunk);
+/* SSIM window of the batched step's loss (default GSR_SSIM_BOX7). */
+int gsr_batch_set_ssim_window(gsr_batch* batch, int window);
 /* Depth-slice passes of the batched step (1 to 8; default 2 with growth 8 for
  * n >= 65536, else 1).  With k > 1 passes the depth-sorted Gaussians are binned
  * in slices whose ends grow by `growth` (2 to 16; the first slice holds

@@ -381,30 +381,54 @@ void forward(const R* splats, const uint32_t* list, const uint32_t* ranges, int 
 template <class R>
 inline R sgn(R v) { return v > R(0) ? R(1) : (v < R(0) ? R(-1) : R(0)); }
 
-// SSIM_COEF (gpu.js:342-372) and LOSS_GRAD (gpu.js:383-409) with lambda > 0;
-// plain L1 when lambda == 0 (gpu.js:750).
+// SSIM window weights per axis: win 7 is the reference's 7x7 box (1/7 each, so
+// 1/49 per tap, gpu.js:360); win 11 an 11x11 Gaussian with sigma 1.5, g
+// normalised to sum 1 (extension).
 template <class R>
-void loss_grad(const R* image, const R* target, int W, int H, R lambda, R* grad) {
+std::vector<R> window_weights(int win) {
+  std::vector<R> w(win);
+  if (win == 7) {
+    for (int k = 0; k < win; ++k) w[k] = R(1) / R(7);
+  } else {
+    R sum = 0;
+    for (int k = 0; k < win; ++k) {
+      R d = R(k - win / 2);
+      w[k] = std::exp(-d * d / (R(2) * R(1.5) * R(1.5)));
+      sum += w[k];
+    }
+    for (int k = 0; k < win; ++k) w[k] /= sum;
+  }
+  return w;
+}
+
+// SSIM_COEF (gpu.js:342-372) and LOSS_GRAD (gpu.js:383-409) with lambda > 0;
+// plain L1 when lambda == 0 (gpu.js:750).  Window win (7: the reference's box,
+// 11: Gaussian): per valid window, dSSIM/dx_p = w_p (a + 2 b x_p + d y_p) with
+// the weighted moments; the gradient sums that over the windows holding p.
+template <class R>
+void loss_grad(const R* image, const R* target, int W, int H, R lambda, R* grad, int win = 7) {
   const size_t P = size_t(W) * H;
+  const int rad = win / 2;
+  const std::vector<R> wt = window_weights<R>(win);
   std::vector<R> coef;
-  const R windows = R(std::max(0, W - 6) * std::max(0, H - 6) * 3);
+  const R windows = R(std::max(0, W - 2 * rad) * std::max(0, H - 2 * rad) * 3);
   if (lambda > R(0)) {
     coef.assign(P * 9, R(0));
 #pragma omp parallel for schedule(static)
     for (long pix = 0; pix < long(P); ++pix) {
       int x = int(pix % W), y = int(pix / W);
-      bool inside = x >= 3 && y >= 3 && x + 3 < W && y + 3 < H;
+      bool inside = x >= rad && y >= rad && x + rad < W && y + rad < H;
       for (int c = 0; c < 3; ++c) {
         R a = 0, b = 0, d = 0;
         if (inside) {
           R mx = 0, my = 0, sxx = 0, syy = 0, sxy = 0;
-          for (int dy = 0; dy < 7; ++dy)
-            for (int dx = 0; dx < 7; ++dx) {
-              size_t q = (size_t(y + dy - 3) * W + x + dx - 3) * 3 + c;
+          for (int dy = 0; dy < win; ++dy)
+            for (int dx = 0; dx < win; ++dx) {
+              size_t q = (size_t(y + dy - rad) * W + x + dx - rad) * 3 + c;
+              R w = wt[dy] * wt[dx];
               R xv = image[q], yv = target[q];
-              mx += xv; my += yv; sxx += xv * xv; syy += yv * yv; sxy += xv * yv;
+              mx += w * xv; my += w * yv; sxx += w * xv * xv; syy += w * yv * yv; sxy += w * xv * yv;
             }
-          mx /= R(49); my /= R(49); sxx /= R(49); syy /= R(49); sxy /= R(49);
           const R c1 = R(0.0001), c2 = R(0.0009);
           R a1 = R(2) * mx * my + c1;
           R a2 = R(2) * (sxy - mx * my) + c2;
@@ -430,14 +454,15 @@ void loss_grad(const R* image, const R* target, int W, int H, R lambda, R* grad)
       R g = (R(1) - lambda) * sgn(xv - yv) / nn;
       if (lambda > R(0) && windows > R(0)) {
         R sa = 0, sb = 0, sd = 0;
-        for (int dy = -3; dy <= 3; ++dy)
-          for (int dx = -3; dx <= 3; ++dx) {
+        for (int dy = -rad; dy <= rad; ++dy)
+          for (int dx = -rad; dx <= rad; ++dx) {
             int cx = x0 + dx, cy = y0 + dy;
             if (cx < 0 || cy < 0 || cx >= W || cy >= H) continue;
             size_t k = (size_t(cy) * W + cx) * 9 + c * 3;
-            sa += coef[k]; sb += coef[k + 1]; sd += coef[k + 2];
+            R w = wt[rad - dy] * wt[rad - dx];
+            sa += w * coef[k]; sb += w * coef[k + 1]; sd += w * coef[k + 2];
           }
-        R dS = (sa + R(2) * xv * sb + yv * sd) / R(49) / windows;
+        R dS = (sa + R(2) * xv * sb + yv * sd) / windows;
         g -= lambda * dS;
       }
       grad[pix * 3 + c] = g;
@@ -445,26 +470,29 @@ void loss_grad(const R* image, const R* target, int W, int H, R lambda, R* grad)
   }
 }
 
-// Loss value as ssim-test.html:13-30 defines it.
+// Loss value as ssim-test.html:13-30 defines it (win 7), or with the 11x11
+// Gaussian window (win 11).
 template <class R>
-R loss_value(const R* image, const R* target, int W, int H, R lambda) {
+R loss_value(const R* image, const R* target, int W, int H, R lambda, int win = 7) {
   const size_t n = size_t(W) * H * 3;
+  const int rad = win / 2;
+  const std::vector<R> wt = window_weights<R>(win);
   R l1 = 0;
   for (size_t i = 0; i < n; ++i) l1 += std::fabs(image[i] - target[i]);
   l1 /= R(n);
   R ssim = 0, windows = 0;
   const R C1 = R(0.0001), C2 = R(0.0009);
-  for (int cy = 3; cy + 3 < H; ++cy)
-    for (int cx = 3; cx + 3 < W; ++cx)
+  for (int cy = rad; cy + rad < H; ++cy)
+    for (int cx = rad; cx + rad < W; ++cx)
       for (int c = 0; c < 3; ++c) {
         R mx = 0, my = 0, sxx = 0, syy = 0, sxy = 0;
-        for (int dy = -3; dy <= 3; ++dy)
-          for (int dx = -3; dx <= 3; ++dx) {
+        for (int dy = -rad; dy <= rad; ++dy)
+          for (int dx = -rad; dx <= rad; ++dx) {
             size_t q = (size_t(cy + dy) * W + cx + dx) * 3 + c;
+            R w = wt[dy + rad] * wt[dx + rad];
             R xv = image[q], yv = target[q];
-            mx += xv; my += yv; sxx += xv * xv; syy += yv * yv; sxy += xv * yv;
+            mx += w * xv; my += w * yv; sxx += w * xv * xv; syy += w * yv * yv; sxy += w * xv * yv;
           }
-        mx /= 49; my /= 49; sxx /= 49; syy /= 49; sxy /= 49;
         ssim += ((R(2) * mx * my + C1) * (R(2) * (sxy - mx * my) + C2)) /
                 ((mx * mx + my * my + C1) * (sxx - mx * mx + syy - my * my + C2));
         windows += 1;
@@ -839,11 +867,11 @@ int oracle_num_threads(int n) {
                             R* state) {                                                                               \
     forward<R>(splats, list, ranges, W, H, image, state);                                                            \
   }                                                                                                                   \
-  void oracle_loss_grad_##SUF(const R* image, const R* target, int W, int H, R lambda, R* grad) {                   \
-    loss_grad<R>(image, target, W, H, lambda, grad);                                                                 \
+  void oracle_loss_grad_##SUF(const R* image, const R* target, int W, int H, R lambda, R* grad, int win) {          \
+    loss_grad<R>(image, target, W, H, lambda, grad, win);                                                            \
   }                                                                                                                   \
-  R oracle_loss_value_##SUF(const R* image, const R* target, int W, int H, R lambda) {                              \
-    return loss_value<R>(image, target, W, H, lambda);                                                               \
+  R oracle_loss_value_##SUF(const R* image, const R* target, int W, int H, R lambda, int win) {                     \
+    return loss_value<R>(image, target, W, H, lambda, win);                                                          \
   }                                                                                                                   \
   void oracle_backward_##SUF(const R* splats, const uint32_t* list, const uint32_t* ranges, const R* state,          \
                              const R* dL, int W, int H, R* grads) {                                                  \

@@ -45,10 +45,11 @@ def lib():
             getattr(_lib, f"oracle_loss_grad_{suf}").argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                                                   ctypes.c_int,
                                                                   ctypes.c_float if suf == "f32" else ctypes.c_double,
-                                                                  ctypes.c_void_p]
+                                                                  ctypes.c_void_p, ctypes.c_int]
             getattr(_lib, f"oracle_loss_value_{suf}").argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                                                    ctypes.c_int,
-                                                                   ctypes.c_float if suf == "f32" else ctypes.c_double]
+                                                                   ctypes.c_float if suf == "f32" else ctypes.c_double,
+                                                                   ctypes.c_int]
     return _lib
 
 
@@ -110,20 +111,21 @@ def forward(splats, lst, ranges, width: int, height: int, dtype=np.float64):
     return image, state
 
 
-def loss_grad(image, target, lam: float, dtype=np.float64):
+def loss_grad(image, target, lam: float, dtype=np.float64, window: int = 7):
+    """dLoss/dpixel; window 7: the reference's 7x7 box SSIM, 11: 11x11 Gaussian."""
     image = _c(image, dtype)
     target = _c(target, dtype)
     H, W = image.shape[:2]
     g = np.zeros_like(image)
-    getattr(lib(), f"oracle_loss_grad_{_suf(dtype)}")(_p(image), _p(target), W, H, lam, _p(g))
+    getattr(lib(), f"oracle_loss_grad_{_suf(dtype)}")(_p(image), _p(target), W, H, lam, _p(g), window)
     return g
 
 
-def loss_value(image, target, lam: float, dtype=np.float64) -> float:
+def loss_value(image, target, lam: float, dtype=np.float64, window: int = 7) -> float:
     image = _c(image, dtype)
     target = _c(target, dtype)
     H, W = image.shape[:2]
-    return float(getattr(lib(), f"oracle_loss_value_{_suf(dtype)}")(_p(image), _p(target), W, H, lam))
+    return float(getattr(lib(), f"oracle_loss_value_{_suf(dtype)}")(_p(image), _p(target), W, H, lam, window))
 
 
 def backward(splats, lst, ranges, state, dL, width: int, height: int, dtype=np.float64, threaded: bool = False):

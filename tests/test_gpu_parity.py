@@ -228,6 +228,20 @@ def test_loss_grad_matches_oracle(lam):
     r.close()
 
 
+@pytest.mark.parametrize("window,size", [(7, (37, 23)), (11, (37, 23)), (11, (130, 70))])
+def test_loss_grad_window_matches_oracle(window, size):
+    W, H = size
+    image, target = lcg_images(W, H)
+    image, target = image.astype(np.float32), target.astype(np.float32)
+    with lib.Context() as ctx:
+        ib, tb, gb = ctx.upload(image), ctx.upload(target), ctx.alloc(image.nbytes)
+        cam = scenes.Camera(view=np.eye(4)[:3].ravel(), fx=1.0, fy=1.0, cx=0.0, cy=0.0, width=W, height=H)
+        ctx.loss_grad(cam, ib, tb, 0.2, gb, window=window)
+        got = ctx.download(gb, np.float32, image.shape)
+    ref = O.loss_grad(image.astype(np.float64), target.astype(np.float64), 0.2, window=window)
+    assert rel_l2(got, ref) < 1e-5
+
+
 # ---------------------------------------------------------------- backward + chain
 
 

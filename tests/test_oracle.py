@@ -261,6 +261,72 @@ def test_loss_ssim_gradient_matches_central_differences():
     assert worst < 1e-4
 
 
+def ssim_by_convolution(image, target, kernel1d):
+    # Wang et al. 2004 SSIM averaged over the valid windows and the channels,
+    # written as separable 2D correlations (scipy) rather than per-window loops.
+    from scipy.signal import correlate2d
+
+    k = np.outer(kernel1d, kernel1d)
+    total, count = 0.0, 0
+    for c in range(3):
+        x, y = image[..., c], target[..., c]
+        mx, my = correlate2d(x, k, "valid"), correlate2d(y, k, "valid")
+        vx = correlate2d(x * x, k, "valid") - mx * mx
+        vy = correlate2d(y * y, k, "valid") - my * my
+        cxy = correlate2d(x * y, k, "valid") - mx * my
+        s = (2 * mx * my + 1e-4) * (2 * cxy + 9e-4) / ((mx * mx + my * my + 1e-4) * (vx + vy + 9e-4))
+        total += s.sum()
+        count += s.size
+    return total / count
+
+
+@pytest.mark.parametrize("window", [7, 11])
+def test_loss_value_matches_convolution_ssim(window):
+    image, target, _ = lcg_pair(26, 19)
+    if window == 7:
+        kernel = np.full(7, 1 / 7)  # ssim-test.html:13-30: 7x7 box
+    else:
+        d = np.arange(11) - 5.0
+        kernel = np.exp(-d * d / (2 * 1.5 ** 2))
+        kernel /= kernel.sum()
+        # the published taps of the normalised sigma-1.5 11-tap SSIM window
+        np.testing.assert_allclose(kernel[:6], [0.00102838, 0.00759876, 0.03600077, 0.10936069, 0.21300554,
+                                                0.26601172], atol=1e-8)
+    lam = 0.2
+    want = (1 - lam) * np.abs(image - target).mean() + lam * (1 - ssim_by_convolution(image, target, kernel))
+    assert abs(O.loss_value(image, target, lam, window=window) - want) < 1e-12
+
+
+@pytest.mark.parametrize("window", [7, 11])
+def test_ssim_gradient_vanishes_at_identity(window):
+    # SSIM(x, x) = 1 is its maximum: the SSIM term's gradient is zero there, and
+    # L1's sign(0) is 0, so the whole gradient vanishes.
+    image, _, _ = lcg_pair(26, 19)
+    assert abs(O.loss_value(image, image, 1.0, window=window)) < 1e-12
+    assert np.abs(O.loss_grad(image, image, 1.0, window=window)).max() < 1e-14
+
+
+def test_loss_gauss11_gradient_matches_central_differences():
+    image, target, rand = lcg_pair(26, 19)
+    lam = 0.2
+    g = O.loss_grad(image, target, lam, window=11).ravel()
+    flat = image.ravel()
+    worst, checked = 0.0, 0
+    for _ in range(40):
+        i = int(rand() * flat.size)
+        if abs(flat[i] - target.ravel()[i]) < 0.01:
+            continue
+        up, dn = flat.copy(), flat.copy()
+        up[i] += 1e-3
+        dn[i] -= 1e-3
+        num = (O.loss_value(up.reshape(image.shape), target, lam, window=11) -
+               O.loss_value(dn.reshape(image.shape), target, lam, window=11)) / 2e-3
+        worst = max(worst, abs(num - g[i]) / max(1e-6, abs(num)))
+        checked += 1
+    assert checked > 30
+    assert worst < 1e-4
+
+
 # ---------------------------------------------------------------- backward and chain
 
 
