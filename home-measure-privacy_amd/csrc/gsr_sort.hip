@@ -204,10 +204,11 @@ void scan_segments(hipStream_t st, uint32_t* data, size_t seg_len, uint32_t segs
 // view's sorted Gaussians; tiles flagged done (already saturated by the slices
 // before) receive no entries.  The count matrix of a pass is laid out
 // [view][chunk][tile] at offset chunks * tile_base of the view.
-constexpr uint32_t kBandTiles = 2048;
+constexpr uint32_t kBandTiles = 2048;      // count kernel: 4 B of LDS per tile
+constexpr uint32_t kEmitBandTiles = 256;   // emit kernel: 12 B of LDS per tile (more resident waves beat fewer rescans)
 
-__host__ __device__ inline uint32_t band_rows(uint32_t tiles_x, uint32_t tiles_y) {
-  uint32_t r = kBandTiles / (tiles_x ? tiles_x : 1);
+__host__ __device__ inline uint32_t band_rows(uint32_t tiles_x, uint32_t tiles_y, uint32_t budget) {
+  uint32_t r = budget / (tiles_x ? tiles_x : 1);
   r = r < 1 ? 1 : r;
   return r < tiles_y ? r : (tiles_y ? tiles_y : 1);
 }
@@ -235,8 +236,8 @@ struct Band {
   bool valid;
 };
 
-__device__ inline Band band_of(const ViewDesc& vd, uint32_t band) {
-  const uint32_t R = band_rows(vd.tiles_x, vd.tiles_y);
+__device__ inline Band band_of(const ViewDesc& vd, uint32_t band, uint32_t budget) {
+  const uint32_t R = band_rows(vd.tiles_x, vd.tiles_y, budget);
   Band b;
   b.tx = vd.tiles_x;
   b.r0 = band * R;
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(64) void bin_count_kernel(const ViewDesc* __restric
   extern __shared__ int diff[];
   const uint32_t v = blockIdx.z, c = blockIdx.x, lane = threadIdx.x;
   const ViewDesc vd = views[v];
-  const Band bd = band_of(vd, blockIdx.y);
+  const Band bd = band_of(vd, blockIdx.y, kBandTiles);
   if (!bd.valid) return;
   const uint32_t rows = bd.r1 - bd.r0, tx = bd.tx, W = tx + 1;
   for (uint32_t k = lane; k < (rows + 1) * W; k += 64) diff[k] = 0;
@@ -369,11 +370,39 @@ __global__ __launch_bounds__(256) void bin_colscan_kernel(const ViewDesc* __rest
   }
 }
 
-// Pass 2: hands out slots from per-tile LDS cursors of the band, seeded with the
-// scanned matrix.  The wave visits its chunk's Gaussians in depth order (only
-// those whose rectangle meets the band, found by ballot); for one Gaussian the
-// lanes cover its tiles in the band, which are distinct, so every tile's slots
-// follow (depth, index).  Bands whose tiles are all done return at once.
+// Pass 2: the order itself.  The block holds per-tile slot cursors of its band,
+// seeded with the scanned matrix, and walks its chunk's Gaussians in depth order,
+// 64 at a time.  The group's (Gaussian, tile) pairs inside the band are
+// flattened Gaussian-major (lane j's pairs occupy [start_j, start_j + cnt_j)) and
+// handed out 64 per batch, one per lane.  Within a batch the pairs of one tile
+// come from distinct Gaussians in lane (= depth) order, so a pair's slot is the
+// tile's cursor plus the number of lower lanes holding the same tile: every lane
+// sets its bit in a 64-bit LDS mask of its tile, reads the mask back and counts
+// the bits below its own; the highest lane of each mask then advances the
+// cursor and clears the mask.  Slots of a tile therefore follow (depth, index).
+// Done tiles carry cursor 0xFFFFFFFF and are skipped; bands whose tiles are all
+// done return at once.
+
+// Inclusive scans over the wave with row shifts and row broadcasts (DPP):
+// rows of 16 first, then row 0's total into row 1 (and row 2's into row 3),
+// then lane 31's running total into rows 2 and 3.
+template <bool kMax>
+__device__ inline uint32_t dpp_step(uint32_t x, uint32_t y) {
+  return kMax ? (x > y ? x : y) : x + y;
+}
+template <bool kMax>
+__device__ inline uint32_t wave_inclusive_scan(uint32_t x) {
+  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x111, 0xF, 0xF, true)));  // row_shr:1
+  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x112, 0xF, 0xF, true)));  // row_shr:2
+  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x114, 0xF, 0xF, true)));  // row_shr:4
+  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x118, 0xF, 0xF, true)));  // row_shr:8
+  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x142, 0xA, 0xF, false)));  // row_bcast:15
+  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x143, 0xC, 0xF, false)));  // row_bcast:31
+  return x;
+}
+
+constexpr uint32_t kEmitFixedLds = 64 * 16 + 64 * 4;  // owner records + owner marks
+
 __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict__ views, uint32_t n, uint32_t chunk,
                                                       uint32_t pbeg, uint32_t pend, uint32_t chunks,
                                                       const uint32_t* __restrict__ vals,
@@ -383,60 +412,92 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
                                                       const uint64_t* __restrict__ ebase_v,
                                                       uint32_t* __restrict__ order, uint64_t capacity,
                                                       uint32_t* __restrict__ overflow) {
-  extern __shared__ uint32_t cursor[];
+  extern __shared__ uint4 lds_quads[];
   const uint32_t v = blockIdx.z, c = blockIdx.x, lane = threadIdx.x;
   const ViewDesc vd = views[v];
-  const Band bd = band_of(vd, blockIdx.y);
+  const Band bd = band_of(vd, blockIdx.y, kEmitBandTiles);
   if (!bd.valid) return;
   const uint32_t tx = bd.tx, ntile = (bd.r1 - bd.r0) * tx;
+  uint4* owner_rec = lds_quads;                                             // [64] start, w, origin, index
+  uint32_t* owner_mark = reinterpret_cast<uint32_t*>(lds_quads + 64);       // [64]
+  uint64_t* mask = reinterpret_cast<uint64_t*>(owner_mark + 64);            // [ntile]
+  uint32_t* cursor = reinterpret_cast<uint32_t*>(mask + ntile);             // [ntile]
   const uint32_t T = tx * vd.tiles_y;
   const uint32_t* base = matrix + size_t(chunks) * vd.tile_base + size_t(c) * T + size_t(bd.r0) * tx;
   const uint8_t* dn = done ? done + vd.tile_base + size_t(bd.r0) * tx : nullptr;
   bool open = false;
   for (uint32_t k = lane; k < ntile; k += 64) {
-    cursor[k] = base[k];
-    open |= !(dn && dn[k]);
+    const bool d = dn && dn[k];
+    cursor[k] = d ? 0xFFFFFFFFu : base[k];
+    mask[k] = 0;
+    open |= !d;
   }
   if (__ballot(open) == 0) return;  // every tile of the band is saturated
-  // Done tiles get cursor 0xFFFFFFFF: their entries are skipped below.
-  if (dn)
-    for (uint32_t k = lane; k < ntile; k += 64)
-      if (dn[k]) cursor[k] = 0xFFFFFFFFu;
-  __syncthreads();
   const size_t seg = size_t(v) * n;
   const uint32_t begin = pbeg + c * chunk, end = min(pend, begin + chunk);
   const uint64_t ebase = ebase_v[v];
+  const uint64_t below = (uint64_t(1) << lane) - 1;
   bool over = false;
+  // The next group's rectangle and index are loaded one group ahead.
+  uint2 rc_next = make_uint2(1u, 0u);  // empty rectangle
+  uint32_t gi_next = 0;
+  if (begin + lane < end) {
+    rc_next = sorted[seg + begin + lane];
+    gi_next = vals[seg + begin + lane];
+  }
+  __syncthreads();
   for (uint32_t p0 = begin; p0 < end; p0 += 64) {
-    const uint32_t p = p0 + lane;
-    Rect r{1u, 0u, 1u, 0u};
-    uint32_t gi = 0;
-    if (p < end) {
-      r = unpack_rect(sorted[seg + p]);
-      gi = vals[seg + p];
+    const Rect r = unpack_rect(rc_next);
+    const uint32_t gi = gi_next;
+    rc_next = make_uint2(1u, 0u);
+    if (p0 + 64 + lane < end) {
+      rc_next = sorted[seg + p0 + 64 + lane];
+      gi_next = vals[seg + p0 + 64 + lane];
     }
     const uint32_t y0 = max(r.y0, bd.r0), y1 = min(r.y1, bd.r1 - 1);
-    uint64_t hits = __ballot(r.x0 <= r.x1 && y0 <= y1);
-    while (hits) {
-      const int j = __ffsll(static_cast<unsigned long long>(hits)) - 1;
-      hits &= hits - 1;
-      const uint32_t jx0 = __builtin_amdgcn_readlane(r.x0, j), jx1 = __builtin_amdgcn_readlane(r.x1, j);
-      const uint32_t jy0 = __builtin_amdgcn_readlane(y0, j) - bd.r0, jy1 = __builtin_amdgcn_readlane(y1, j) - bd.r0;
-      const uint32_t jgi = __builtin_amdgcn_readlane(gi, j);
-      const uint32_t w = jx1 - jx0 + 1, cnt = w * (jy1 - jy0 + 1);
-      const float rcp_w = 1.0f / float(w);
-      for (uint32_t t = lane; t < cnt; t += 64) {
-        uint32_t row = uint32_t(float(t) * rcp_w);
-        while (row * w > t) --row;
-        while ((row + 1) * w <= t) ++row;
-        const uint32_t idx = (jy0 + row) * tx + jx0 + (t - row * w);
-        const uint32_t pos = cursor[idx];
-        if (pos == 0xFFFFFFFFu) continue;  // done tile
-        cursor[idx] = pos + 1;
-        const uint64_t slot = ebase + pos;
-        if (slot < capacity) order[slot] = jgi;
-        else over = true;
+    const bool hit = r.x0 <= r.x1 && y0 <= y1;
+    const uint32_t w = hit ? r.x1 - r.x0 + 1 : 1u;
+    const uint32_t cnt = hit ? w * (y1 - y0 + 1) : 0u;
+    const uint32_t incl = wave_inclusive_scan<false>(cnt);
+    const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+    if (total == 0) continue;
+    const uint32_t start = incl - cnt;
+    owner_rec[lane] = make_uint4(start, w, hit ? (y0 - bd.r0) * tx + r.x0 : 0u, gi);
+    for (uint32_t b0 = 0; b0 < total; b0 += 64) {
+      // Mark where each Gaussian's pairs begin in this batch (lane + 1), then a
+      // running max hands every pair its Gaussian.
+      owner_mark[lane] = 0;
+      if (cnt > 0 && start < b0 + 64 && incl > b0) owner_mark[(start > b0 ? start : b0) - b0] = lane + 1;
+      __syncthreads();
+      const uint32_t own = wave_inclusive_scan<true>(owner_mark[lane]) - 1;
+      const uint32_t e = b0 + lane;
+      const bool act = e < total;
+      uint32_t t = 0, pos = 0xFFFFFFFFu, idx = 0;
+      if (act) {
+        const uint4 o = owner_rec[own];
+        const uint32_t k = e - o.x, ow = o.y;
+        uint32_t row = uint32_t(float(k) * (1.0f / float(ow)));
+        while (row * ow > k) --row;
+        while ((row + 1) * ow <= k) ++row;
+        t = o.z + row * tx + (k - row * ow);
+        idx = o.w;
+        pos = cursor[t];
+        atomicOr(reinterpret_cast<unsigned long long*>(&mask[t]), static_cast<unsigned long long>(1) << lane);
       }
+      __syncthreads();
+      if (act) {
+        const uint64_t m = mask[t];
+        if (pos != 0xFFFFFFFFu) {
+          const uint64_t slot = ebase + pos + uint32_t(__popcll(m & below));
+          if (slot < capacity) order[slot] = idx;
+          else over = true;
+        }
+        if ((m >> lane) == 1) {  // highest lane of this tile in the batch
+          if (pos != 0xFFFFFFFFu) cursor[t] = pos + uint32_t(__popcll(m));
+          mask[t] = 0;
+        }
+      }
+      __syncthreads();
     }
   }
   if (over) atomicOr(overflow, 1u);
@@ -484,7 +545,7 @@ size_t bin_matrix_elems(const ViewDesc* host_views, uint32_t num_views, uint32_t
 
 namespace {
 struct BandGeometry {
-  uint32_t max_bands = 1, max_tiles = 1;
+  uint32_t max_bands = 1, max_emit_bands = 1, max_tiles = 1;
   size_t count_lds = 4, emit_lds = 4;
 };
 
@@ -492,11 +553,12 @@ BandGeometry band_geometry(const ViewDesc* host_views, uint32_t num_views) {
   BandGeometry g;
   for (uint32_t v = 0; v < num_views; ++v) {
     const uint32_t tx = host_views[v].tiles_x, ty = host_views[v].tiles_y;
-    const uint32_t R = band_rows(tx, ty);
+    const uint32_t R = band_rows(tx, ty, kBandTiles), Re = band_rows(tx, ty, kEmitBandTiles);
     g.max_bands = std::max(g.max_bands, (ty + R - 1) / R);
+    g.max_emit_bands = std::max(g.max_emit_bands, (ty + Re - 1) / Re);
     g.max_tiles = std::max(g.max_tiles, tx * ty);
     g.count_lds = std::max(g.count_lds, size_t(R + 1) * (tx + 1) * sizeof(int));
-    g.emit_lds = std::max(g.emit_lds, size_t(R) * tx * sizeof(uint32_t));
+    g.emit_lds = std::max(g.emit_lds, kEmitFixedLds + size_t(Re) * tx * (sizeof(uint64_t) + sizeof(uint32_t)));
   }
   return g;
 }
@@ -538,7 +600,8 @@ void launch_bin_emit(hipStream_t s, const ViewDesc* views, const ViewDesc* host_
   const BandGeometry g = band_geometry(host_views, num_views);
   dim3 tgrid((g.max_tiles + 255) / 256, num_views);
   hipLaunchKernelGGL(bin_colscan_kernel, tgrid, dim3(256), 0, s, views, chunks, pass.ranges, bs.matrix);
-  hipLaunchKernelGGL(bin_emit_kernel, dim3(chunks, g.max_bands, num_views), dim3(64), g.emit_lds, s, views, n, chunk,
+  hipLaunchKernelGGL(bin_emit_kernel, dim3(chunks, g.max_emit_bands, num_views), dim3(64), g.emit_lds, s, views, n,
+                     chunk,
                      pass.pbeg, pass.pend, chunks, vals, bs.rect_sorted, pass.done, bs.matrix, pass.ebase, order,
                      order_capacity, overflow);
 }
