@@ -8,66 +8,16 @@
 // opacityLogit, color[3]} packed as 16 floats (gpu.js:688-694); cameras {view[12],
 // fx, fy, cx, cy, width, height}, sent as the 80-byte uniform (gpu.js:696-701).
 // Optional SH: gaussians[i].sh = 48 floats ([16][3], coefficient 0 = colour).
+// The device-free helpers (packing, .ply / .splat, densify, ...) live in
+// formats.js and are re-exported here.
 
 const path = require('path');
 const addon = require(path.join(__dirname, 'gsr.node'));
 
+const F = require(path.join(__dirname, 'formats.js'));
+const { packGaussians, unpackGaussians, cameraUniform, sceneExtent, shuffled, densify } = F;
+
 const GSR_E_CAPACITY = -4;
-
-/** packGaussians (gpu.js:688-694): 16 floats per Gaussian. */
-function packGaussians(gaussians) {
-  const out = new Float32Array(gaussians.length * 16);
-  for (let i = 0; i < gaussians.length; i++) {
-    const g = gaussians[i];
-    const o = i * 16;
-    out[o] = g.mean[0]; out[o + 1] = g.mean[1]; out[o + 2] = g.mean[2];
-    out[o + 4] = g.logScale[0]; out[o + 5] = g.logScale[1]; out[o + 6] = g.logScale[2];
-    out[o + 7] = g.opacityLogit;
-    for (let k = 0; k < 4; k++) out[o + 8 + k] = g.rotation[k];
-    out[o + 12] = g.color[0]; out[o + 13] = g.color[1]; out[o + 14] = g.color[2];
-  }
-  return out;
-}
-
-/** unpackGaussians (gpu.js:602-609). */
-function unpackGaussians(data) {
-  const out = [];
-  for (let i = 0; i < data.length / 16; i++) {
-    const o = i * 16;
-    out.push({
-      mean: [data[o], data[o + 1], data[o + 2]],
-      logScale: [data[o + 4], data[o + 5], data[o + 6]],
-      opacityLogit: data[o + 7],
-      rotation: [data[o + 8], data[o + 9], data[o + 10], data[o + 11]],
-      color: [data[o + 12], data[o + 13], data[o + 14]],
-    });
-  }
-  return out;
-}
-
-/** The 80-byte camera uniform (cameraUniform, gpu.js:696-701). */
-function cameraUniform(cam, count) {
-  const buf = new ArrayBuffer(80);
-  const f = new Float32Array(buf);
-  for (let k = 0; k < 12; k++) f[k] = cam.view[k];
-  f[12] = cam.fx || 0; f[13] = cam.fy || 0; f[14] = cam.cx || 0; f[15] = cam.cy || 0;
-  f[16] = cam.width; f[17] = cam.height;
-  const u = new Uint32Array(buf);
-  u[18] = count; u[19] = cam.width * cam.height;
-  return buf;
-}
-
-/** sceneExtent (gpu.js:564-568): 1.1 x the largest camera distance from their mean. */
-function sceneExtent(views) {
-  const centres = views.map((v) => {
-    const m = v.cam.view;
-    return [0, 1, 2].map((j) => -(m[j] * m[3] + m[4 + j] * m[7] + m[8 + j] * m[11]));
-  });
-  const mean = [0, 1, 2].map((j) => centres.reduce((a, c) => a + c[j], 0) / centres.length);
-  let far = 1e-3;
-  for (const c of centres) far = Math.max(far, Math.hypot(c[0] - mean[0], c[1] - mean[1], c[2] - mean[2]));
-  return 1.1 * far;
-}
 
 let sharedCtx = null;
 function defaultCtx() {
@@ -126,24 +76,42 @@ async function createRenderer({ gpuTiles = true, device = 0 } = {}) { // eslint-
   const buffer = (bytes) => addon.alloc(ctx, Math.max(16, bytes));
   const release = (bufs) => { for (const b of bufs) if (b !== null && b !== undefined) addon.free(ctx, b); };
 
-  // project -> tile lists -> blend (forward(), gpu.js:783-812).
-  function forward(gBuf, shBuf, shDegree, n, cam) {
+  // Buffers reused between training steps, grown by a quarter when too small
+  // (scratch(), gpu.js:764-772).
+  const pool = new Map();
+  function scratch(name, bytes) {
+    const have = pool.get(name);
+    if (have && have.bytes >= bytes) return have.ptr;
+    if (have) addon.free(ctx, have.ptr);
+    const size = Math.max(16, Math.ceil(bytes * 1.25 / 4) * 4);
+    const ptr = addon.alloc(ctx, size);
+    pool.set(name, { ptr, bytes: size });
+    return ptr;
+  }
+  function pooled(name) { return pool.get(name).bytes; }
+
+  // project -> tile lists -> blend (forward(), gpu.js:783-812).  With `reuse`
+  // the buffers come from the pool (training) and stay there.
+  function forward(gBuf, shBuf, shDegree, n, cam, reuse = false) {
+    const get = reuse ? scratch : (name, bytes) => buffer(bytes);
     const camBuf = cameraUniform(cam, n);
-    const sBuf = buffer(n * 48);
+    const sBuf = get('splats', n * 48);
     addon.project(ctx, camBuf, gBuf, shBuf, shDegree, n, sBuf);
     const tiles = Math.ceil(cam.width / 16) * Math.ceil(cam.height / 16);
-    const rBuf = buffer(tiles * 8);
+    const rBuf = get('ranges', tiles * 8);
+    let oBuf = reuse ? scratch('order', 4) : null;
     let entries;
     try {
-      entries = addon.binSort(ctx, camBuf, sBuf, n, null, 0, rBuf);
+      entries = addon.binSort(ctx, camBuf, sBuf, n, oBuf, reuse ? pooled('order') / 4 : 0, rBuf);
     } catch (e) {
       if (e.code !== GSR_E_CAPACITY) throw e;
       entries = e.entries;
+      oBuf = get('order', entries * 4);
+      if (entries > 0) addon.binSort(ctx, camBuf, sBuf, n, oBuf, entries, rBuf);
     }
-    const oBuf = buffer(entries * 4);
-    if (entries > 0) addon.binSort(ctx, camBuf, sBuf, n, oBuf, entries, rBuf);
+    if (oBuf === null) oBuf = buffer(4);
     const pixels = cam.width * cam.height;
-    const iBuf = buffer(pixels * 12);
+    const iBuf = get('image', pixels * 12);
     addon.forward(ctx, camBuf, sBuf, oBuf, rBuf, iBuf, null);
     return { camBuf, sBuf, oBuf, rBuf, iBuf, pixels, entries };
   }
@@ -253,7 +221,96 @@ async function createRenderer({ gpuTiles = true, device = 0 } = {}) { // eslint-
     return { gaussians: unpackGaussians(data), losses };
   }
 
-  return { device: ctx, render, gradients, train, lossGradientOf };
+  /**
+   * trainScene (gpu.js:890-959): one view per step, every view once per round in
+   * a new order; densify every `densifyEvery` steps in [densifyFrom,
+   * densifyUntil]; `onProgress({step, loss, count})` every 50 steps (awaited);
+   * `shouldStop()` ends early.  Views are {cam, image} with image the photo's RGB
+   * bytes.  `random` (extension) replaces Math.random for the shuffles and splits.
+   */
+  async function trainScene(start, views, { steps = 3000, densifyEvery = 100, densifyFrom = 300, densifyUntil = 2000,
+                                            maxSplats = 400000, ssim = 0, prune = false, opacityReset = false,
+                                            onProgress, shouldStop, lr, random = Math.random } = {}) {
+    lr = Object.assign({ color: 0.025, opacity: 0.05, mean: 0.002, scale: 0.005, rotation: 0.001 }, lr);
+    let gaussians = start;
+    let n = gaussians.length;
+    const extent = sceneExtent(views);
+    const resetAt = opacityReset ? Math.round((densifyFrom + densifyUntil) / 2 / densifyEvery) * densifyEvery : -1;
+    const lowOpacity = Math.log(0.01 / 0.99);
+    let gBuf = null, moments = null, statsBuf = null;
+    const upload = () => {
+      release([gBuf, moments, statsBuf]);
+      n = gaussians.length;
+      gBuf = buffer(n * 64);
+      addon.upload(ctx, gBuf, packGaussians(gaussians));
+      moments = buffer(n * 28 * 4);
+      statsBuf = buffer(n * 8);
+    };
+    upload();
+    const stepData = new ArrayBuffer(32);
+    let lossSum = 0, lossCount = 0;
+    let order = [];
+    try {
+      for (let t = 1; t <= steps; t++) {
+        if (shouldStop && shouldStop()) break;
+        if (!order.length) order = shuffled(views.length, random);
+        const view = views[order.pop()];
+        const f = forward(gBuf, null, 0, n, view.cam, true);
+        const tBuf = scratch('target', f.pixels * 12);
+        addon.upload(ctx, tBuf, Float32Array.from(view.image, (v) => v / 255));
+        const gradBuf = scratch('grads', n * 64);
+        addon.clear(ctx, gradBuf, n * 64);
+        const lg = scratch('lossGrad', f.pixels * 12);
+        addon.lossGrad(ctx, f.camBuf, f.iBuf, tBuf, ssim, lg);
+        addon.backward(ctx, f.camBuf, f.sBuf, f.oBuf, f.rBuf, f.iBuf, lg, gradBuf);
+        const shapeBuf = scratch('shape', n * 40);
+        addon.shape(ctx, f.camBuf, gBuf, null, 0, n, gradBuf, shapeBuf, null);
+        new Float32Array(stepData).set([t, 0, lr.color, lr.opacity, lr.mean, lr.scale, lr.rotation, 0]);
+        new Uint32Array(stepData)[1] = n;
+        addon.stats(ctx, stepData, gradBuf, f.sBuf, statsBuf);
+        addon.adam(ctx, stepData, gBuf, gradBuf, shapeBuf, moments);
+        if (t % 25 === 0) {
+          const image = new Float32Array(f.pixels * 3);
+          addon.download(ctx, f.iBuf, image);
+          let loss = 0;
+          for (let i = 0; i < image.length; i++) loss += Math.abs(image[i] - view.image[i] / 255);
+          lossSum += loss / image.length;
+          lossCount++;
+        }
+        if (t >= densifyFrom && t <= densifyUntil && t % densifyEvery === 0) {
+          // per-pixel gradients: the 3DGS threshold (NDC units) scaled by half the width
+          const gradThreshold = 2e-4 / (0.5 * views[0].cam.width);
+          const data = new Float32Array(n * 16);
+          const stats = new Float32Array(n * 2);
+          addon.download(ctx, gBuf, data);
+          addon.download(ctx, statsBuf, stats);
+          gaussians = densify(unpackGaussians(data), stats, { gradThreshold, maxCount: maxSplats,
+            sizeThreshold: 0.01 * extent, maxSize: prune ? 0.1 * extent : Infinity }, random);
+          if (t === resetAt) for (const g of gaussians) g.opacityLogit = Math.min(g.opacityLogit, lowOpacity);
+          upload();
+        }
+        if (t % 50 === 0) {
+          if (onProgress) await onProgress({ step: t, loss: lossCount ? lossSum / lossCount : NaN, count: n });
+          lossSum = 0;
+          lossCount = 0;
+          await new Promise((r) => setTimeout(r));
+        }
+      }
+      const data = new Float32Array(n * 16);
+      addon.download(ctx, gBuf, data);
+      return unpackGaussians(data);
+    } finally {
+      release([gBuf, moments, statsBuf]);
+      for (const entry of pool.values()) addon.free(ctx, entry.ptr);
+      pool.clear();
+    }
+  }
+
+  return { device: ctx, render, gradients, train, trainScene, lossGradientOf };
 }
 
-module.exports = { createRenderer, packGaussians, unpackGaussians, tileLists, sceneExtent, cameraUniform };
+module.exports = {
+  createRenderer, tileLists, packGaussians, unpackGaussians, cameraUniform, sceneExtent, shuffled, densify,
+  initFromPoints: F.initFromPoints, neighbourSizes: F.neighbourSizes, toPly: F.toPly, fromPly: F.fromPly,
+  toSplat: F.toSplat,
+};
