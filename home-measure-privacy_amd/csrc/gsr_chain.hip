@@ -257,65 +257,69 @@ __device__ inline float adam_update(float g, float lr, float bc1, float bc2, flo
   return lr * (m / bc1) / (sqrtf(v / bc2) + 1e-8f);
 }
 
-__global__ __launch_bounds__(256) void adam_sh_kernel(gsr_step st, float lr_sh_rest, float* __restrict__ gs,
-                                                      float* __restrict__ sh, const float* __restrict__ grads,
-                                                      const float* __restrict__ shape,
-                                                      const float* __restrict__ sh_grad,
-                                                      float* __restrict__ moments) {
+// Adam over the 59 numbers of an SH3 Gaussian in two coalesced sweeps.  Moments
+// (120 n floats): SH m [n][48] at 0, SH v [n][48] at 48 n, then the 11 base
+// parameters (mean 0-2, log-scale 3-5, rotation 6-9, opacity logit 10) m at
+// 96 n + k n + i and v at 107 n + k n + i; the last 2 n floats are unused.
+__global__ __launch_bounds__(256) void adam_base_kernel(gsr_step st, float* __restrict__ gs,
+                                                        const float* __restrict__ grads,
+                                                        const float* __restrict__ shape,
+                                                        float* __restrict__ moments) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= st.count) return;
+  const uint32_t n = st.count;
+  if (i >= n) return;
   const float bc1 = 1.0f - powf(0.9f, st.t);
   const float bc2 = 1.0f - powf(0.999f, st.t);
-  float* b = gs + size_t(i) * 16;
-  float4* m4 = reinterpret_cast<float4*>(moments + size_t(i) * 120);
-  // Parameters 0-10 from the 16-float record (slots 0-2, 4-6, 8-11, 7).
-  {
-    const int slot[11] = {0, 1, 2, 4, 5, 6, 8, 9, 10, 11, 7};
-    float m[12], v[12];
+  float4* rec = reinterpret_cast<float4*>(gs) + size_t(i) * 4;
+  float4 a = rec[0], b = rec[1], c = rec[2];
+  float* mb = moments + size_t(96) * n + i;
+  float* vb = moments + size_t(107) * n + i;
+  const float* shp = shape + size_t(i) * 10;
+  float* p[11] = {&a.x, &a.y, &a.z, &b.x, &b.y, &b.z, &c.x, &c.y, &c.z, &c.w, &b.w};
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const float4 a = m4[q], c = m4[15 + q];
-      m[4 * q] = a.x; m[4 * q + 1] = a.y; m[4 * q + 2] = a.z; m[4 * q + 3] = a.w;
-      v[4 * q] = c.x; v[4 * q + 1] = c.y; v[4 * q + 2] = c.z; v[4 * q + 3] = c.w;
-    }
-    const float* shp = shape + size_t(i) * 10;
-#pragma unroll
-    for (int k = 0; k < 11; ++k) {
-      const float g = k < 10 ? shp[k] : grads[size_t(i) * 16 + 3];
-      const float lr = k < 3 ? st.lr_mean : k < 6 ? st.lr_scale : k < 10 ? st.lr_rotation : st.lr_opacity;
-      b[slot[k]] -= adam_update(g, lr, bc1, bc2, m[k], v[k]);
-    }
-    // parameter 11 (SH 0, red DC) shares the third float4 with 8-10
-    const float g11 = sh_grad[size_t(i) * 48];
-    const float u11 = adam_update(g11, st.lr_color, bc1, bc2, m[11], v[11]);
-    sh[size_t(i) * 48] -= u11;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      m4[q] = make_float4(m[4 * q], m[4 * q + 1], m[4 * q + 2], m[4 * q + 3]);
-      m4[15 + q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
-    }
+  for (int k = 0; k < 11; ++k) {
+    const float g = k < 10 ? shp[k] : grads[size_t(i) * 16 + 3];
+    const float lr = k < 3 ? st.lr_mean : k < 6 ? st.lr_scale : k < 10 ? st.lr_rotation : st.lr_opacity;
+    float m = mb[size_t(k) * n], v = vb[size_t(k) * n];
+    *p[k] -= adam_update(g, lr, bc1, bc2, m, v);
+    mb[size_t(k) * n] = m;
+    vb[size_t(k) * n] = v;
   }
-  // SH parameters 12-58 (coefficients 1..47 of the [16][3] block).
-  float* s = sh + size_t(i) * 48;
-  const float* gsh = sh_grad + size_t(i) * 48;
-  for (int q = 3; q < 15; ++q) {
-    float4 a = m4[q], c = m4[15 + q];
-    float m[4] = {a.x, a.y, a.z, a.w}, v[4] = {c.x, c.y, c.z, c.w};
+  const float len = fmaxf(sqrtf(c.x * c.x + c.y * c.y + c.z * c.z + c.w * c.w), 1e-12f);
+  c = make_float4(c.x / len, c.y / len, c.z / len, c.w / len);
+  rec[0] = a;
+  rec[1] = b;
+  rec[2] = c;
+}
+
+// SH coefficients: element e of sh [n][48] is coefficient (e mod 48) / 3; the DC
+// triple uses lr_color, bands 1-3 lr_sh_rest.  Four elements per thread.
+__global__ __launch_bounds__(256) void adam_sh_coef_kernel(gsr_step st, float lr_sh_rest, float* __restrict__ sh,
+                                                           const float* __restrict__ sh_grad,
+                                                           float* __restrict__ moments) {
+  const size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t quads = size_t(st.count) * 12;
+  if (j >= quads) return;
+  const float bc1 = 1.0f - powf(0.9f, st.t);
+  const float bc2 = 1.0f - powf(0.999f, st.t);
+  float4* s4 = reinterpret_cast<float4*>(sh) + j;
+  float4* m4 = reinterpret_cast<float4*>(moments) + j;
+  float4* v4 = reinterpret_cast<float4*>(moments) + quads + j;
+  float4 x = *s4, m = *m4, v = *v4;
+  const float4 g = reinterpret_cast<const float4*>(sh_grad)[j];
+  const uint32_t e0 = uint32_t(j % 12) * 4;  // element of the first lane within its Gaussian's 48
+  float* xs[4] = {&x.x, &x.y, &x.z, &x.w};
+  float* ms[4] = {&m.x, &m.y, &m.z, &m.w};
+  float* vs[4] = {&v.x, &v.y, &v.z, &v.w};
+  const float gg[4] = {g.x, g.y, g.z, g.w};
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int k = 4 * q + e;  // parameter index; SH coefficient k - 11
-      if (k > 58) break;
-      const int c3 = k - 11;
-      const float lr = c3 < 3 ? st.lr_color : lr_sh_rest;  // DC: lr_color, bands 1-3: lr_sh_rest
-      s[c3] -= adam_update(gsh[c3], lr, bc1, bc2, m[e], v[e]);
-    }
-    m4[q] = make_float4(m[0], m[1], m[2], m[3]);
-    m4[15 + q] = make_float4(v[0], v[1], v[2], v[3]);
+  for (int c = 0; c < 4; ++c) {
+    const float lr = e0 + c < 3 ? st.lr_color : lr_sh_rest;
+    *xs[c] -= adam_update(gg[c], lr, bc1, bc2, *ms[c], *vs[c]);
   }
-  float qv[4] = {b[8], b[9], b[10], b[11]};
-  float len = fmaxf(sqrtf(qv[0] * qv[0] + qv[1] * qv[1] + qv[2] * qv[2] + qv[3] * qv[3]), 1e-12f);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) b[8 + k] = qv[k] / len;
+  *s4 = x;
+  *m4 = m;
+  *v4 = v;
 }
 
 // STATS.main (gpu.js:491-497).
@@ -347,8 +351,11 @@ void launch_adam(hipStream_t s, const gsr_step& step, float* gs, const float* gr
 void launch_adam_sh(hipStream_t s, const gsr_step& step, float lr_sh_rest, float* gs, float* sh, const float* grads,
                     const float* shape, const float* sh_grad, float* moments) {
   if (step.count == 0) return;
-  hipLaunchKernelGGL(adam_sh_kernel, dim3((step.count + 255) / 256), dim3(256), 0, s, step, lr_sh_rest, gs, sh, grads,
-                     shape, sh_grad, moments);
+  hipLaunchKernelGGL(adam_base_kernel, dim3((step.count + 255) / 256), dim3(256), 0, s, step, gs, grads, shape,
+                     moments);
+  const size_t quads = size_t(step.count) * 12;
+  hipLaunchKernelGGL(adam_sh_coef_kernel, dim3(uint32_t((quads + 255) / 256)), dim3(256), 0, s, step, lr_sh_rest, sh,
+                     sh_grad, moments);
 }
 
 void launch_stats(hipStream_t s, const gsr_step& step, const float* grads, const float* splats, float* stats) {

@@ -154,10 +154,11 @@ int gsr_adam(gsr_ctx* ctx, const gsr_step* step, float* gs, const float* grads, 
  * SH training): mean (lr_mean), log-scale (lr_scale), rotation (lr_rotation),
  * opacity logit (lr_opacity, gradient grads[3]), SH DC (lr_color) and SH bands
  * 1-3 (lr_sh_rest), with the reference's update and quaternion renormalisation;
- * SH coefficients are not clamped.  gs, sh [n][16][3] and moments [n][120]
- * (m of parameter k at k, v at 60 + k; parameters mean 0-2, logScale 3-5,
- * rotation 6-9, opacity 10, SH 11-58) are updated in place; sh_grad as written
- * by gsr_shape / gsr_batch_step.  sh and moments are 16-byte aligned. */
+ * SH coefficients are not clamped.  gs, sh [n][16][3] and moments (120 n
+ * floats, zero at the start: SH m [n][48] at 0 and v at 48 n; parameter k of
+ * mean 0-2, logScale 3-5, rotation 6-9, opacity 10 has m at 96 n + k n + i and
+ * v at 107 n + k n + i) are updated in place; sh_grad as written by gsr_shape /
+ * gsr_batch_step.  sh, sh_grad and moments are 16-byte aligned. */
 int gsr_adam_sh(gsr_ctx* ctx, const gsr_step* step, float lr_sh_rest, float* gs, float* sh, const float* grads,
                 const float* shape, const float* sh_grad, float* moments);
 

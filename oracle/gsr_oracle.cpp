@@ -759,8 +759,9 @@ void shape_fd(const CamU* cu, const R* gs, uint32_t n, const R* grads, R* shape)
 }
 
 // Adam over the 59 numbers of an SH3 Gaussian (the extension of gpu.js:411-443):
-// parameters mean 0-2, logScale 3-5, rotation 6-9, opacity logit 10, SH 11-58;
-// moments [n][120] with m at k and v at 60 + k.  The reference's update per
+// parameters mean 0-2, logScale 3-5, rotation 6-9, opacity logit 10, SH 11-58.
+// Moments (120 n): SH m [n][48] at 0 and v at 48 n; base parameter k < 11 m at
+// 96 n + k n + i and v at 107 n + k n + i.  The reference's update per
 // parameter, then the quaternion renormalisation; SH coefficients unclamped.
 template <class R>
 void adam_sh(const StepU* st, float lr_sh_rest, float* gs, float* sh, const R* grads, const R* shape,
@@ -770,10 +771,11 @@ void adam_sh(const StepU* st, float lr_sh_rest, float* gs, float* sh, const R* g
   const R t = R(st->t);
   const R bc1 = R(1) - std::pow(R(0.9), t), bc2 = R(1) - std::pow(R(0.999), t);
   for (uint32_t i = 0; i < n; ++i) {
-    R* mo = moments + size_t(i) * 120;
     for (int k = 0; k < 59; ++k) {
       R g, lr;
       float* p;
+      R* mk = k < 11 ? moments + size_t(96) * n + size_t(k) * n + i : moments + size_t(i) * 48 + (k - 11);
+      R* vk = k < 11 ? moments + size_t(107) * n + size_t(k) * n + i : moments + size_t(48) * n + size_t(i) * 48 + (k - 11);
       if (k < 10) {
         g = shape[i * 10 + k];
         lr = R(k < 3 ? st->lrMean : k < 6 ? st->lrScale : st->lrRotation);
@@ -787,10 +789,10 @@ void adam_sh(const StepU* st, float lr_sh_rest, float* gs, float* sh, const R* g
         lr = R(k - 11 < 3 ? st->lrColor : lr_sh_rest);
         p = sh + i * 48 + (k - 11);
       }
-      R m = R(0.9) * mo[k] + R(0.1) * g;
-      R v = R(0.999) * mo[60 + k] + R(0.001) * g * g;
-      mo[k] = m;
-      mo[60 + k] = v;
+      R m = R(0.9) * *mk + R(0.1) * g;
+      R v = R(0.999) * *vk + R(0.001) * g * g;
+      *mk = m;
+      *vk = v;
       *p = float(R(*p) - lr * (m / bc1) / (std::sqrt(v / bc2) + R(1e-8)));
     }
     float* b = gs + i * 16;

@@ -499,10 +499,14 @@ def test_adam_sh_closed_form():
     rates = np.full((16, 3), 2.5e-4)
     rates[0] = lr["color"]
     np.testing.assert_allclose(sh1, sh - step(shg, rates[None]), atol=1e-6)
-    np.testing.assert_allclose(mom[:, 10], 0.1 * grads[:, 3], rtol=1e-12)
-    np.testing.assert_allclose(mom[:, 11:59], 0.1 * shg.reshape(n, 48), rtol=1e-12)
-    np.testing.assert_allclose(mom[:, 71:119], 0.001 * shg.reshape(n, 48) ** 2, rtol=1e-12)
-    assert not mom[:, 59].any() and not mom[:, 119].any()
+    flat = mom.ravel()
+    m_sh, v_sh = flat[:48 * n].reshape(n, 48), flat[48 * n:96 * n].reshape(n, 48)
+    m_base, v_base = flat[96 * n:107 * n].reshape(11, n), flat[107 * n:118 * n].reshape(11, n)
+    np.testing.assert_allclose(m_base[10], 0.1 * grads[:, 3], rtol=1e-12)
+    np.testing.assert_allclose(v_base[0:3].T, 0.001 * shape[:, 0:3] ** 2, rtol=1e-12)
+    np.testing.assert_allclose(m_sh, 0.1 * shg.reshape(n, 48), rtol=1e-12)
+    np.testing.assert_allclose(v_sh, 0.001 * shg.reshape(n, 48) ** 2, rtol=1e-12)
+    assert not flat[118 * n:].any()
     # untouched slots: pad 3, colour 12-14 (SH0 colour is not an SH3 parameter), 15
     np.testing.assert_array_equal(gs1[:, [3, 12, 13, 14, 15]], gs[:, [3, 12, 13, 14, 15]])
 

@@ -4,7 +4,7 @@
    ABI from Python (same view order, same learning rates): the trained Gaussians
    agree to the nondeterminism of the backward's float atomics.
 2. With densification, the splat count grows at the first densify step and stays
-   within maxSplats, and the photometric loss falls.  The densify rules
+   within maxSplats, and the L1 over all views falls.  The densify rules
    themselves are checked on the CPU (test_formats.py).
 """
 import json
@@ -116,6 +116,11 @@ def test_train_scene_densifies_and_learns():
     assert counts[0] == start.shape[0] and counts[1] > counts[0]
     assert counts[-1] <= 5000 and counts[-1] == done["count"] == trained.shape[0]
     assert np.isfinite(trained).all()
-    assert prog[-1]["loss"] < 0.8 * prog[0]["loss"]
+    # photometric L1 over all views, rendered before and after
+    r = Renderer()
+    l1 = lambda gs: np.mean([np.abs(r.render(gs, c) - img / 255.0).mean() for c, img in zip(cams, images)])  # noqa: E731
+    before, after = l1(start), l1(trained)
+    r.close()
+    assert after < 0.8 * before
     print(f"trainScene: {opts['steps'] / done['seconds']:.1f} steps/s at 96x96, counts {counts}, "
-          f"loss {prog[0]['loss']:.4f} -> {prog[-1]['loss']:.4f}", flush=True)
+          f"L1 over all views {before:.4f} -> {after:.4f}", flush=True)
