@@ -372,16 +372,18 @@ __global__ __launch_bounds__(256) void bin_colscan_kernel(const ViewDesc* __rest
 
 // Pass 2: the order itself.  The block holds per-tile slot cursors of its band,
 // seeded with the scanned matrix, and walks its chunk's Gaussians in depth order,
-// 64 at a time.  The group's (Gaussian, tile) pairs inside the band are
-// flattened Gaussian-major (lane j's pairs occupy [start_j, start_j + cnt_j)) and
-// handed out 64 per batch, one per lane.  Within a batch the pairs of one tile
-// come from distinct Gaussians in lane (= depth) order, so a pair's slot is the
-// tile's cursor plus the number of lower lanes holding the same tile: every lane
-// sets its bit in a 64-bit LDS mask of its tile, reads the mask back and counts
-// the bits below its own; the highest lane of each mask then advances the
-// cursor and clears the mask.  Slots of a tile therefore follow (depth, index).
-// Done tiles carry cursor 0xFFFFFFFF and are skipped; bands whose tiles are all
-// done return at once.
+// 64 at a time.  Gaussians whose rectangle meets the band join an LDS queue, in
+// order; every 64 queued Gaussians (and the rest at the end) are flushed.  A
+// flush flattens their (Gaussian, tile) pairs Gaussian-major (queue entry j's
+// pairs occupy [start_j, start_j + cnt_j)) and hands them out 64 per batch, one
+// per lane.  Within a batch the pairs of one tile come from distinct Gaussians
+// in queue (= depth) order, so a pair's slot is the tile's cursor plus the
+// number of lower lanes holding the same tile: every lane sets its bit in a
+// 64-bit LDS mask of its tile, reads the mask back and counts the bits below its
+// own; the highest lane of each mask then advances the cursor and clears the
+// mask.  Slots of a tile therefore follow (depth, index).  Done tiles carry
+// cursor 0xFFFFFFFF and are skipped; bands whose tiles are all done return at
+// once.
 
 // Inclusive scans over the wave with row shifts and row broadcasts (DPP):
 // rows of 16 first, then row 0's total into row 1 (and row 2's into row 3),
@@ -401,7 +403,8 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t x) {
   return x;
 }
 
-constexpr uint32_t kEmitFixedLds = 64 * 16 + 64 * 4;  // owner records + owner marks
+constexpr uint32_t kEmitQueue = 128;
+constexpr uint32_t kEmitFixedLds = kEmitQueue * 16 + 64 * 4;  // queue records + owner marks
 
 __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict__ views, uint32_t n, uint32_t chunk,
                                                       uint32_t pbeg, uint32_t pend, uint32_t chunks,
@@ -418,10 +421,12 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
   const Band bd = band_of(vd, blockIdx.y, kEmitBandTiles);
   if (!bd.valid) return;
   const uint32_t tx = bd.tx, ntile = (bd.r1 - bd.r0) * tx;
-  uint4* owner_rec = lds_quads;                                             // [64] start, w, origin, index
-  uint32_t* owner_mark = reinterpret_cast<uint32_t*>(lds_quads + 64);       // [64]
-  uint64_t* mask = reinterpret_cast<uint64_t*>(owner_mark + 64);            // [ntile]
-  uint32_t* cursor = reinterpret_cast<uint32_t*>(mask + ntile);             // [ntile]
+  // queue record: band-local tile of the rectangle's corner, width, pair count
+  // (replaced by the first pair's position during a flush), Gaussian index
+  uint4* queue = lds_quads;                                                  // [kEmitQueue]
+  uint32_t* owner_mark = reinterpret_cast<uint32_t*>(lds_quads + kEmitQueue);  // [64]
+  uint64_t* mask = reinterpret_cast<uint64_t*>(owner_mark + 64);             // [ntile]
+  uint32_t* cursor = reinterpret_cast<uint32_t*>(mask + ntile);              // [ntile]
   const uint32_t T = tx * vd.tiles_y;
   const uint32_t* base = matrix + size_t(chunks) * vd.tile_base + size_t(c) * T + size_t(bd.r0) * tx;
   const uint8_t* dn = done ? done + vd.tile_base + size_t(bd.r0) * tx : nullptr;
@@ -438,34 +443,21 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
   const uint64_t ebase = ebase_v[v];
   const uint64_t below = (uint64_t(1) << lane) - 1;
   bool over = false;
-  // The next group's rectangle and index are loaded one group ahead.
-  uint2 rc_next = make_uint2(1u, 0u);  // empty rectangle
-  uint32_t gi_next = 0;
-  if (begin + lane < end) {
-    rc_next = sorted[seg + begin + lane];
-    gi_next = vals[seg + begin + lane];
-  }
-  __syncthreads();
-  for (uint32_t p0 = begin; p0 < end; p0 += 64) {
-    const Rect r = unpack_rect(rc_next);
-    const uint32_t gi = gi_next;
-    rc_next = make_uint2(1u, 0u);
-    if (p0 + 64 + lane < end) {
-      rc_next = sorted[seg + p0 + 64 + lane];
-      gi_next = vals[seg + p0 + 64 + lane];
-    }
-    const uint32_t y0 = max(r.y0, bd.r0), y1 = min(r.y1, bd.r1 - 1);
-    const bool hit = r.x0 <= r.x1 && y0 <= y1;
-    const uint32_t w = hit ? r.x1 - r.x0 + 1 : 1u;
-    const uint32_t cnt = hit ? w * (y1 - y0 + 1) : 0u;
+  uint32_t queued = 0;  // wave-uniform
+
+  // Emits the pairs of queue entries [0, count) and drops them from the queue.
+  auto flush = [&](uint32_t count) {
+    __syncthreads();
+    uint4 rec = make_uint4(0u, 1u, 0u, 0u);
+    if (lane < count) rec = queue[lane];
+    const uint32_t cnt = rec.z;
     const uint32_t incl = wave_inclusive_scan<false>(cnt);
     const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-    if (total == 0) continue;
     const uint32_t start = incl - cnt;
-    owner_rec[lane] = make_uint4(start, w, hit ? (y0 - bd.r0) * tx + r.x0 : 0u, gi);
+    if (lane < count) queue[lane].z = start;
     for (uint32_t b0 = 0; b0 < total; b0 += 64) {
-      // Mark where each Gaussian's pairs begin in this batch (lane + 1), then a
-      // running max hands every pair its Gaussian.
+      // Mark where each entry's pairs begin in this batch (lane + 1), then a
+      // running max hands every pair its entry.
       owner_mark[lane] = 0;
       if (cnt > 0 && start < b0 + 64 && incl > b0) owner_mark[(start > b0 ? start : b0) - b0] = lane + 1;
       __syncthreads();
@@ -474,12 +466,12 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
       const bool act = e < total;
       uint32_t t = 0, pos = 0xFFFFFFFFu, idx = 0;
       if (act) {
-        const uint4 o = owner_rec[own];
-        const uint32_t k = e - o.x, ow = o.y;
+        const uint4 o = queue[own];
+        const uint32_t k = e - o.z, ow = o.y;
         uint32_t row = uint32_t(float(k) * (1.0f / float(ow)));
         while (row * ow > k) --row;
         while ((row + 1) * ow <= k) ++row;
-        t = o.z + row * tx + (k - row * ow);
+        t = o.x + row * tx + (k - row * ow);
         idx = o.w;
         pos = cursor[t];
         atomicOr(reinterpret_cast<unsigned long long*>(&mask[t]), static_cast<unsigned long long>(1) << lane);
@@ -499,7 +491,67 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
       }
       __syncthreads();
     }
+    const uint32_t rest = queued - count;  // < 64
+    uint4 moved = make_uint4(0u, 0u, 0u, 0u);
+    if (lane < rest) moved = queue[count + lane];
+    __syncthreads();
+    if (lane < rest) queue[lane] = moved;
+    queued = rest;
+    __syncthreads();
+  };
+
+  // Rectangles and indices are loaded kAhead groups at a time, one load round
+  // ahead: stores count in vmcnt too, so the wait for a round also waits for the
+  // flushes' stores, and rounds make that wait rarer.
+  constexpr int kAhead = 4;
+  uint2 nrc[kAhead];
+  uint32_t ngi[kAhead];
+  auto load_round = [&](uint32_t q0) {
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) {
+      const uint32_t p = q0 + 64 * u + lane;
+      nrc[u] = make_uint2(1u, 0u);  // empty rectangle
+      ngi[u] = 0;
+      if (p < end) {
+        nrc[u] = sorted[seg + p];
+        ngi[u] = vals[seg + p];
+      }
+    }
+  };
+  load_round(begin);
+  for (uint32_t q0 = begin; q0 < end; q0 += 64 * kAhead) {
+    uint2 crc[kAhead];
+    uint32_t cgi[kAhead];
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) {
+      crc[u] = nrc[u];
+      cgi[u] = ngi[u];
+    }
+    if (q0 + 64 * kAhead < end) load_round(q0 + 64 * kAhead);
+#pragma unroll 1
+    for (int u = 0; u < kAhead; ++u) {
+      const Rect r = unpack_rect(crc[0]);
+      const uint32_t gi = cgi[0];
+#pragma unroll
+      for (int k = 0; k + 1 < kAhead; ++k) {
+        crc[k] = crc[k + 1];
+        cgi[k] = cgi[k + 1];
+      }
+      const uint32_t y0 = max(r.y0, bd.r0), y1 = min(r.y1, bd.r1 - 1);
+      const bool hit = r.x0 <= r.x1 && y0 <= y1;
+      const uint64_t hits = __ballot(hit);
+      if (hits == 0) continue;
+      if (hit) {
+        const uint32_t rank =
+            __builtin_amdgcn_mbcnt_hi(uint32_t(hits >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(hits), 0u));
+        const uint32_t w = r.x1 - r.x0 + 1;
+        queue[queued + rank] = make_uint4((y0 - bd.r0) * tx + r.x0, w, w * (y1 - y0 + 1), gi);
+      }
+      queued += uint32_t(__popcll(hits));
+      if (queued >= 64) flush(64);
+    }
   }
+  if (queued > 0) flush(queued);
   if (over) atomicOr(overflow, 1u);
 }
 
