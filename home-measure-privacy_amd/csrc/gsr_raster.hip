@@ -491,9 +491,10 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
         // Same alpha, transmittance and accumulated colour as forward_tile, bit for bit.
         const float alpha = alpha_at(dy, q, row, op, rad, T[i]);
         const bool h = alpha != 0.0f;
-        // No pixel of the band hit: T, colour and gradients stay as they are.
-        if (__ballot(h) == 0) continue;
-        hit = true;
+        // A miss (alpha 0) leaves T and the accumulated colour bit for bit as they
+        // are, so the state is updated unconditionally (this form compiles
+        // without copies of the pixel state); the gradient terms only where a
+        // pixel of the band was hit.
         const float Tk = T[i];
         const float wgt = alpha * Tk;
         const float oma = 1.0f - alpha;
@@ -501,6 +502,8 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
         ar[i] = fmaf(r2.x, wgt, ar[i]);
         ag[i] = fmaf(r2.y, wgt, ag[i]);
         ab[i] = fmaf(r2.z, wgt, ab[i]);
+        if (__ballot(h) == 0) continue;
+        hit = true;
         // gpu.js:266-268: colour gradient and dLoss/dalpha = d . (Tk c - behind / (1 - alpha)).
         g[0] = fmaf(dr[i], wgt, g[0]);
         g[1] = fmaf(dg[i], wgt, g[1]);
