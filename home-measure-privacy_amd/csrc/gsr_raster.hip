@@ -113,14 +113,6 @@ __device__ inline float2 entry_extent(const Batch& b) {
   return hit_extent(b.r0.z, b.r0.w, b.r1.x, b.r1.y, b.r1.z);
 }
 
-// LDS layout of a staged entry: (u, v, A, B), (C, opacity, radius, ex), (r, g, b, ey).
-__device__ inline void stage(float4 (*lds)[3], uint32_t lane, const Batch& b) {
-  const float2 e = entry_extent(b);
-  lds[lane][0] = b.r0;
-  lds[lane][1] = make_float4(b.r1.x, b.r1.y, b.r1.z, e.x);
-  lds[lane][2] = make_float4(b.r2.x, b.r2.y, b.r2.z, e.y);
-}
-
 // Per-entry conic coefficients of the exponent in base 2:
 // power * log2(e) = qA dx^2 + dy (qC dy - qB dx) with qA = -A/2 log2 e,
 // qB = B log2 e, qC = -C/2 log2 e.  The sign (the power > 0 test) is unchanged.
@@ -201,25 +193,38 @@ __device__ inline bool band_culled(float ylo, float v, float ey) {
   return (ylo + 3.0f - v) < -ey || (ylo - v) > ey;
 }
 
-// Lane rows (lane >> 4) + 4 i form band i: tile rows 4 i .. 4 i + 3, pixel
-// centres band0 + 4 i .. band0 + 4 i + 3.  The bands an entry's hit box (|dy| <=
-// ey around v) can reach are [lo, hi]: band i needs 4 i + 3 >= v - ey - band0
-// and 4 i <= v + ey - band0.  The bounds are widened by a margin covering the
-// float32 rounding of these sums, so a band outside them holds no pixel whose
-// own offset py - v is within ey.  Returned in scalar registers (hi < lo: none).
-struct Bands {
-  int lo, hi;
-};
+// LDS layout of a staged entry: (u, v, A, B), (C, opacity, radius, ex), (r, g, b,
+// ey) and, for the backward (kQuads == 4), (qA, qB, qC, bands).  The fourth quad
+// holds what depends only on the entry and the tile, computed once per entry by
+// its own lane rather than by the whole wave in the walk: the base-2 conic and
+// the bands of the tile (bit i: rows 4 i .. 4 i + 3, pixel centres band0 + 4 i
+// ..) that the hit box reaches.  A band whose bit is clear holds no pixel the
+// entry can hit.  (The forward recomputes these in its walk: there the extra
+// LDS read per entry costs more than it saves.)
+template <int kQuads>
+__device__ inline void stage(float4 (*lds)[kQuads], uint32_t lane, const Batch& b, float band0) {
+  const float2 e = entry_extent(b);
+  lds[lane][0] = b.r0;
+  lds[lane][1] = make_float4(b.r1.x, b.r1.y, b.r1.z, e.x);
+  lds[lane][2] = make_float4(b.r2.x, b.r2.y, b.r2.z, e.y);
+  if constexpr (kQuads == 4) {
+    const Conic2 q = conic2(b.r0.z, b.r0.w, b.r1.x);
+    uint32_t bands = 0;
+#pragma unroll
+    for (int i = 0; i < kPix; ++i) bands |= band_culled(band0 + 4.0f * i, b.r0.y, e.y) ? 0u : (1u << i);
+    lds[lane][3] = make_float4(q.qA, q.qB, q.qC, __uint_as_float(bands));
+  }
+}
 
-__device__ inline Bands band_range(float band0, float v, float ey) {
-  const float margin = 1e-3f + 1e-6f * (fabsf(v) + fabsf(ey) + fabsf(band0));
-  const float flo = ceilf((((v - ey) - band0) - 3.0f - margin) * 0.25f);
-  const float fhi = floorf((((v + ey) - band0) + margin) * 0.25f);
-  // clamp to [-1, 4] before the conversion (NaN -> empty)
-  const int lo = int(fminf(fmaxf(flo, -1.0f), 4.0f));
-  const int hi = int(fminf(fmaxf(fhi, -1.0f), 4.0f));
-  const bool ok = flo == flo && fhi == fhi;
-  return Bands{__builtin_amdgcn_readfirstlane(ok ? lo : 4), __builtin_amdgcn_readfirstlane(ok ? hi : -1)};
+// Bands [first, last] of a work item as a bit mask.
+__device__ inline uint32_t item_bands(const Tile& t) {
+  return ((1u << (t.last + 1)) - 1u) & ~((1u << t.first) - 1u);
+}
+
+// The staged band bits of entry j (backward staging), as a scalar, restricted to
+// the item's bands.
+__device__ inline uint32_t entry_bands(const float4 (*lds)[4], uint32_t j, uint32_t item) {
+  return __builtin_amdgcn_readfirstlane(__float_as_uint(lds[j][3].w)) & item;
 }
 
 // ---- forward -------------------------------------------------------------------
@@ -265,7 +270,7 @@ __device__ __forceinline__ void forward_tile(const RasterArgs& a, uint32_t w, fl
       break;
     }
     __syncthreads();
-    stage(lds, lane, nb);
+    stage<3>(lds, lane, nb, band0);
     __syncthreads();
     if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
     const uint32_t nk = min(64u, t.end - k0);
@@ -422,7 +427,7 @@ __device__ inline int reduce9_slot(uint32_t lane) {
   return lane == 2 ? 8 : -1;
 }
 
-__device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3], uint32_t* gi_lds) {
+__device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[4], uint32_t* gi_lds) {
   const uint32_t lane = __lane_id();
   Tile t;
   if (!load_tile(a, w, t)) return;
@@ -451,6 +456,7 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
     }
   }
   const float band0 = float(t.y0) + 0.5f;
+  const uint32_t items = item_bands(t);
   float* grads = a.grads + size_t(t.view) * a.grads_view_stride;
   // The passes of a tile follow each other in depth order; pixel state carries over.
   for (int p = 0; p < a.num_passes; ++p) {
@@ -459,7 +465,7 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
   for (uint32_t k0 = t.start; k0 < t.end; k0 += 64) {
     if (!any_active(T)) break;
     __syncthreads();
-    stage(lds, lane, nb);
+    stage<4>(lds, lane, nb, band0);
     gi_lds[lane] = nb.gi;
     __syncthreads();
     if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
@@ -470,10 +476,12 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
         if (!any_active(T)) break;
         box = active_box(px, py0, T);
       }
-      const float4 r0 = lds[j][0], r1 = lds[j][1], r2 = lds[j][2];
+      const uint32_t bands = entry_bands(lds, j, items);
+      if (bands == 0) continue;
+      const float4 r0 = lds[j][0], r1 = lds[j][1], r2 = lds[j][2], r3 = lds[j][3];
       const float u = r0.x, v = r0.y, A = r0.z, B = r0.w, C = r1.x, op = r1.y, rad = r1.z;
       if (box_culled(box, u, v, r1.w, r2.w)) continue;
-      const Conic2 q = conic2(A, B, C);
+      const Conic2 q{r3.x, r3.y, r3.z};
       const float dx = px - u;
       const Row row = row_terms(q, dx, rad);
       const float omo = 1.0f - op;
@@ -482,11 +490,9 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
 #pragma unroll
       for (int m = 0; m < 9; ++m) g[m] = 0.0f;
       bool hit = false;
-      const Bands bands = band_range(band0, v, r2.w);
-      const int blo = max(bands.lo, t.first), bhi = min(bands.hi, t.last);
 #pragma unroll
       for (int i = 0; i < kPix; ++i) {
-        if (i < blo || i > bhi) continue;
+        if (((bands >> i) & 1u) == 0) continue;
         const float dy = (py0 + 4.0f * i) - v;
         // Same alpha, transmittance and accumulated colour as forward_tile, bit for bit.
         const float alpha = alpha_at(dy, q, row, op, rad, T[i]);
@@ -532,13 +538,13 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
 }
 
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void backward_kernel(RasterArgs a) {
-  __shared__ float4 lds[64][3];
+  __shared__ float4 lds[64][4];
   __shared__ uint32_t gi_lds[64];
   backward_tile(a, blockIdx.x, lds, gi_lds);
 }
 
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void backward_persistent_kernel(RasterArgs a) {
-  __shared__ float4 lds[64][3];
+  __shared__ float4 lds[64][4];
   __shared__ uint32_t gi_lds[64];
   const uint32_t items = num_items(a);
   for (;;) {
