@@ -159,23 +159,43 @@ struct Box {
   float x0, x1, y0, y1;
 };
 
-__device__ inline Box active_box(float px, float py0, const float T[kPix]) {
-  float ylo = __builtin_huge_valf(), yhi = -__builtin_huge_valf();
+// From the four per-slot ballots and scalar bit scans: lane l of slot i is the
+// pixel (x0 + (l & 15), y0 + (l >> 4) + 4 i), so the first / last active column
+// is the lowest / highest set bit of the masks folded to 16 bits, and the first
+// / last active row lies in the first / last slot with an active pixel, at its
+// lowest / highest set lane.  The coordinates are the pixels' own centres
+// (float(x) + 0.5, float(y0 + (l >> 4)) + 0.5 + 4 i), as the walk computes them.
+__device__ inline Box active_box(uint32_t x0, uint32_t y0, const float T[kPix]) {
+  uint64_t m[kPix];
+  uint64_t all = 0;
+#pragma unroll
+  for (int i = 0; i < kPix; ++i) {
+    m[i] = __ballot(T[i] > kTStop);
+    all |= m[i];
+  }
+  if (all == 0) return Box{__builtin_huge_valf(), -__builtin_huge_valf(), __builtin_huge_valf(), -__builtin_huge_valf()};
+  const uint32_t half = uint32_t(all) | uint32_t(all >> 32);
+  const uint32_t cols = (half | (half >> 16)) & 0xFFFFu;
+  const uint32_t c0 = uint32_t(__builtin_ctz(cols)), c1 = uint32_t(31 - __builtin_clz(cols));
+  int ifirst = kPix - 1, ilast = 0;
+#pragma unroll
+  for (int i = kPix - 1; i >= 0; --i)
+    if (m[i] != 0) ifirst = i;
 #pragma unroll
   for (int i = 0; i < kPix; ++i)
-    if (T[i] > kTStop) {
-      ylo = fminf(ylo, py0 + 4.0f * i);
-      yhi = fmaxf(yhi, py0 + 4.0f * i);
-    }
-  const bool any = ylo <= yhi;
-  Box b{any ? px : __builtin_huge_valf(), any ? px : -__builtin_huge_valf(), ylo, yhi};
+    if (m[i] != 0) ilast = i;
+  uint64_t mf = m[kPix - 1], ml = m[0];
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    b.x0 = fminf(b.x0, __shfl_xor(b.x0, off));
-    b.x1 = fmaxf(b.x1, __shfl_xor(b.x1, off));
-    b.y0 = fminf(b.y0, __shfl_xor(b.y0, off));
-    b.y1 = fmaxf(b.y1, __shfl_xor(b.y1, off));
+  for (int i = 0; i < kPix; ++i) {
+    if (i == ifirst) mf = m[i];
+    if (i == ilast) ml = m[i];
   }
+  const uint32_t r0 = uint32_t(__builtin_ctzll(mf)) >> 4, r1 = uint32_t(63 - __builtin_clzll(ml)) >> 4;
+  Box b;
+  b.x0 = float(x0 + c0) + 0.5f;
+  b.x1 = float(x0 + c1) + 0.5f;
+  b.y0 = (float(y0 + r0) + 0.5f) + 4.0f * float(ifirst);
+  b.y1 = (float(y0 + r1) + 0.5f) + 4.0f * float(ilast);
   return b;
 }
 
@@ -274,11 +294,11 @@ __device__ __forceinline__ void forward_tile(const RasterArgs& a, uint32_t w, fl
     __syncthreads();
     if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
     const uint32_t nk = min(64u, t.end - k0);
-    Box box = active_box(px, py0, T);
+    Box box = active_box(t.x0, t.y0, T);
     for (uint32_t j = 0; j < nk; ++j) {
       if ((j & 15) == 0 && j != 0) {
         if (!any_active(T)) break;
-        if ((j & 31) == 0) box = active_box(px, py0, T);  // a stale box is a superset: still exact
+        if ((j & 31) == 0) box = active_box(t.x0, t.y0, T);  // a stale box is a superset: still exact
       }
       if (kLast) walked = k0 + j + 1;
       const float4 r0 = lds[j][0], r1 = lds[j][1], r2 = lds[j][2];
@@ -470,11 +490,11 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
     __syncthreads();
     if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
     const uint32_t nk = min(64u, t.end - k0);
-    Box box = active_box(px, py0, T);
+    Box box = active_box(t.x0, t.y0, T);
     for (uint32_t j = 0; j < nk; ++j) {
       if ((j & 15) == 0 && j != 0) {
         if (!any_active(T)) break;
-        box = active_box(px, py0, T);
+        box = active_box(t.x0, t.y0, T);
       }
       const uint32_t bands = entry_bands(lds, j, items);
       if (bands == 0) continue;
