@@ -153,7 +153,7 @@ def copy_calibration(mib: int):
 
 
 def train_loop(args):
-    """Config 4: 3M Gaussians (64 clustered blobs), SH3, 64 views at 1600x1066; each
+    """The training workload: 3M Gaussians (64 clustered blobs), SH3, 64 views at 1600x1066; each
     step is the batched forward + backward of all views (L1) followed by the fused
     SH3 Adam on the summed gradients.  Steps are timed one by one (synchronised) for
     the p50 / p99 step time."""

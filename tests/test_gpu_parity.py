@@ -89,21 +89,21 @@ def test_project_matches_oracle(ctx, deg):
 # ---------------------------------------------------------------- tile lists
 
 
-def test_bin_sort_bit_exact_cfg0(ctx):
+def test_bin_sort_bit_exact_10k_256px(ctx):
     sc = scenes.config(0, with_targets=False)
     cam = sc.cams[0]
     splats = gpu_project(ctx, cam, sc.gs)
-    log(f"bin cfg0 n={sc.n}")
+    log(f"bin 10k n={sc.n}")
     order, _ = check_bin_exact(ctx, cam, splats)
     assert order.size > 30000
 
 
-def test_bin_sort_bit_exact_cfg1_full_size(ctx):
+def test_bin_sort_bit_exact_300k_1280x720(ctx):
     """300k SH3 Gaussians at 1280x720: every entry of the ordering matches."""
     sc = scenes.config(1, with_targets=False)
     cam = sc.cams[0]
     splats = gpu_project(ctx, cam, sc.gs, sc.sh, 3)
-    log(f"bin cfg1 n={sc.n} {cam.width}x{cam.height}")
+    log(f"bin 300k n={sc.n} {cam.width}x{cam.height}")
     order, _ = check_bin_exact(ctx, cam, splats)
     log(f"  entries {order.size}")
     assert order.size > 10 * sc.n
@@ -160,8 +160,8 @@ def test_bin_sort_capacity_error(ctx):
 # ---------------------------------------------------------------- forward
 
 
-def render_inputs(ctx, cfg=0, n=None, crop=None):
-    sc = scenes.config(cfg, n=n)
+def render_inputs(ctx, scene=0, n=None, crop=None):
+    sc = scenes.config(scene, n=n)
     cam = sc.cams[0] if crop is None else sc.cams[0].crop(*crop)
     splats = gpu_project(ctx, cam, sc.gs, sc.sh, sc.sh_degree)
     order, ranges = check_bin_exact(ctx, cam, splats)
@@ -179,10 +179,10 @@ def gpu_forward(ctx, cam, splats, order, ranges):
     return img, state
 
 
-@pytest.mark.parametrize("cfg,n,crop", [(0, None, None), (2, 200000, (800, 400, 320, 200))])
-def test_forward_matches_oracle(ctx, cfg, n, crop):
-    sc, cam, splats, order, ranges = render_inputs(ctx, cfg, n, crop)
-    log(f"forward cfg{cfg} E={order.size}")
+@pytest.mark.parametrize("scene,n,crop", [(0, None, None), (2, 200000, (800, 400, 320, 200))])
+def test_forward_matches_oracle(ctx, scene, n, crop):
+    sc, cam, splats, order, ranges = render_inputs(ctx, scene, n, crop)
+    log(f"forward scene {scene} E={order.size}")
     img, state = gpu_forward(ctx, cam, splats, order, ranges)
     i32, s32 = O.forward(splats, order, ranges, cam.width, cam.height, np.float32)
     i64, s64 = O.forward(splats.astype(np.float64), order, ranges, cam.width, cam.height, np.float64)
@@ -260,14 +260,14 @@ def gpu_backward(ctx, cam, splats, order, ranges, image, dl):
 GROUPS = {"colour": slice(0, 3), "opacity": slice(3, 4), "uv": slice(4, 6), "conic": slice(6, 9)}
 
 
-@pytest.mark.parametrize("cfg,n,crop", [(0, None, None), (2, 4000, (896, 476, 128, 128)),
+@pytest.mark.parametrize("scene,n,crop", [(0, None, None), (2, 4000, (896, 476, 128, 128)),
                                         (2, 300000, (700, 300, 256, 160))])
-def test_backward_matches_fp64_oracle(ctx, cfg, n, crop):
-    sc, cam, splats, order, ranges = render_inputs(ctx, cfg, n, crop)
+def test_backward_matches_fp64_oracle(ctx, scene, n, crop):
+    sc, cam, splats, order, ranges = render_inputs(ctx, scene, n, crop)
     img, _ = gpu_forward(ctx, cam, splats, order, ranges)
     rng = np.random.default_rng(3)
     dl = rng.standard_normal(img.shape).astype(np.float32) / img.size
-    log(f"backward cfg{cfg} E={order.size}")
+    log(f"backward scene {scene} E={order.size}")
     got = gpu_backward(ctx, cam, splats, order, ranges, img, dl)
     s64 = splats.astype(np.float64)
     i64, st64 = O.forward(s64, order, ranges, cam.width, cam.height, np.float64)
@@ -278,7 +278,7 @@ def test_backward_matches_fp64_oracle(ctx, cfg, n, crop):
         e64 = rel_l2(got[:, sl], ref[:, sl])
         e32 = rel_l2(ref32[:, sl], ref[:, sl])
         log(f"  {name}: gpu vs fp64 {e64:.2e}   float32 restatement vs fp64 {e32:.2e}")
-        # 1e-4 relative L2 against the FP64 oracle
+        # the bar: 1e-4 relative L2 against the FP64 oracle
         assert e64 < 1e-4, name
     assert not got[:, 9:].any()
 
@@ -311,7 +311,7 @@ def test_shape_chain_matches_fp64_oracle(ctx, deg):
 
 
 def test_gradients_end_to_end_crop(ctx):
-    """cfg 2 differentiation check: 4k Gaussians, 128x128 crop, SH3, full chain
+    """Differentiation check on the 1M-Gaussian SH3 scene: 4k Gaussians, 128x128 crop, full chain
     from (mean, log-scale, rotation, opacity logit, SH) vs the FP64 oracle run from
     the same parameters (its own projection, tile lists and blend)."""
     sc = scenes.config(2, n=4000)
@@ -337,7 +337,7 @@ def test_gradients_end_to_end_crop(ctx):
 
 
 def test_gradients_match_central_differences_64_params(ctx):
-    """cfg 2 differentiation check: GPU gradients (SH3, 4k Gaussians, 128x128 crop)
+    """Differentiation check on the 1M-Gaussian SH3 scene: GPU gradients (4k Gaussians, 128x128 crop)
     against FP64 central differences of the loss on 64 random parameters (means,
     log-scales, rotations, opacity logits, SH coefficients).  h = 1e-7: at h = 1e-3
     the differences cross the renderer's discrete rules (ceil'd radius, 1/255

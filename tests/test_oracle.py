@@ -394,11 +394,11 @@ def test_shape_chain_matches_projection_differences():
             assert num == pytest.approx(shape[i, k], rel=1e-6, abs=1e-6 * (1 + abs(shape[i, k])))
 
 
-@pytest.mark.parametrize("cfg", [0, 2])
-def test_full_chain_matches_central_differences(cfg):
+@pytest.mark.parametrize("scene", [0, 2])
+def test_full_chain_matches_central_differences(scene):
     """Every parameter group (mean, log-scale, rotation, opacity logit, colour or
     SH) through project -> tile lists -> blend, FP64, h = 1e-7."""
-    if cfg == 0:
+    if scene == 0:
         sc, cam = fd_scene()
     else:
         sc = scenes.config(2, n=4000)
