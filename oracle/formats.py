@@ -1,6 +1,6 @@
 """CPU restatement of the host-side formats of trainer/gpu.js, in Python.
 
-TEST INFRASTRUCTURE ONLY: the checker for home-measure-privacy_amd/js/formats.js.
+TEST INFRASTRUCTURE ONLY: what home-measure-privacy_amd/js/formats.js is tested against.
 Imported by tests/ only; never by the product package.
 
 Gaussians are dicts with the reference's keys (gpu.js:512-518); arithmetic is

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/gsr_oracle.cpp) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/gsr_oracle.cpp) before the GPU results are compared with it.
 
 Each check is independent of the oracle's own formulas: closed forms, invariants,
 brute force written separately here, known answers from the reference's tests
