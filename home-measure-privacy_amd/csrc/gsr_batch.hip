@@ -631,6 +631,31 @@ int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const flo
   }
   int rc = gsr_batch_configure(b, best_tb, best_ch);
   if (rc) return rc;
+  // Then the walk above which the backward blends a tile one band per wave: low
+  // enough to shorten the launch's tail, high enough not to repeat each entry's
+  // staging four times over for tiles that are merely long (scenes of dense
+  // clusters prefer a higher one).
+  const uint32_t splits[] = {1024, 2048, 4096};
+  double best_split_t = 1e30;
+  uint32_t best_split = b->split_backward;
+  for (uint32_t sp : splits) {
+    b->split_backward = sp;
+    drop_graph(b);  // the threshold is a launch argument of the captured step
+    rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
+    if (rc) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < 3; ++k) {
+      rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
+      if (rc) return rc;
+    }
+    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (dt < best_split_t) {
+      best_split_t = dt;
+      best_split = sp;
+    }
+  }
+  b->split_backward = best_split;
+  drop_graph(b);
   if (tile_batch) *tile_batch = best_tb;
   if (chunk) *chunk = best_ch;
   return GSR_OK;
