@@ -61,13 +61,22 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
   hist[(size_t(v) * kBins + threadIdx.x) * chunks + c] = h[threadIdx.x];
 }
 
-// Stable scatter of one chunk: wave w owns keys [w*1024, (w+1)*1024) of the chunk
-// and ranks them in 16 rounds of 64 with ballot peer masks.
+// Stable scatter of one chunk.  Wave w owns keys [w*1024, (w+1)*1024) of the
+// chunk and ranks them in 16 rounds of 64 with ballot peer masks; the chunk is
+// then reordered by digit in LDS (stable: wave, round, lane order is index
+// order) and written out digit run by digit run, so consecutive threads store
+// to consecutive addresses.
+__device__ inline uint32_t block_exclusive_scan256(uint32_t x, uint32_t* tmp);
+
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t n, uint32_t chunks,
     int shift, const uint32_t* __restrict__ hist, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out) {
-  __shared__ uint32_t wcnt[4][kBins];
-  __shared__ uint32_t gpos[4][kBins];
+  __shared__ uint32_t wcnt[4][kBins];  // per-wave digit counts, then the waves' local digit bases
+  __shared__ uint32_t lstart[kBins];   // first local slot of each digit
+  __shared__ uint32_t gpos[kBins];     // first output slot of each digit for this chunk
+  __shared__ uint32_t tmp[kSortThreads];
+  __shared__ uint32_t lkeys[kSortChunk];
+  __shared__ uint32_t lvals[kSortChunk];
   const uint32_t v = blockIdx.y, c = blockIdx.x;
   const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
 #pragma unroll
@@ -95,21 +104,34 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   __syncthreads();
   {
     const uint32_t d = threadIdx.x;
-    uint32_t g = hist[(size_t(v) * kBins + d) * chunks + c];
-    uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d];
-    gpos[0][d] = g;
-    gpos[1][d] = g + c0;
-    gpos[2][d] = g + c0 + c1;
-    gpos[3][d] = g + c0 + c1 + c2;
+    const uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d], c3 = wcnt[3][d];
+    const uint32_t ls = block_exclusive_scan256(c0 + c1 + c2 + c3, tmp);
+    lstart[d] = ls;
+    gpos[d] = hist[(size_t(v) * kBins + d) * chunks + c];
+    wcnt[0][d] = ls;
+    wcnt[1][d] = ls + c0;
+    wcnt[2][d] = ls + c0 + c1;
+    wcnt[3][d] = ls + c0 + c1 + c2;
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     if (dl[r] == 0xFFFFFFFFu) continue;
-    uint32_t d = dl[r] >> 16, local = dl[r] & 0xFFFFu;
-    uint32_t dst = gpos[w][d] + local;
-    keys_out[seg + dst] = kr[r];
-    vals_out[seg + dst] = vr[r];
+    const uint32_t local = wcnt[w][dl[r] >> 16] + (dl[r] & 0xFFFFu);
+    lkeys[local] = kr[r];
+    lvals[local] = vr[r];
+  }
+  __syncthreads();
+  const uint32_t len = min(uint32_t(kSortChunk), n - c * kSortChunk);
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const uint32_t li = uint32_t(r) * kSortThreads + threadIdx.x;
+    if (li >= len) continue;
+    const uint32_t key = lkeys[li];
+    const uint32_t d = (key >> shift) & (kBins - 1);
+    const uint32_t dst = gpos[d] + (li - lstart[d]);
+    keys_out[seg + dst] = key;
+    vals_out[seg + dst] = lvals[li];
   }
 }
 
@@ -133,6 +155,12 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* tmp, uint3
   total = tmp[blockDim.x - 1];
   __syncthreads();
   return incl - x;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ inline uint32_t block_exclusive_scan256(uint32_t x, uint32_t* tmp) {
+  uint32_t total;
+  return block_exclusive_scan(x, tmp, total);
 }
 
 __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* __restrict__ data, size_t seg_len,
