@@ -416,8 +416,10 @@ int gsr_adam_sh(gsr_ctx* ctx, const gsr_step* step, float lr_sh_rest, float* gs,
   if (!ctx) return GSR_E_INVALID;
   if (!step || (step->count && (!gs || !sh || !grads || !shape || !sh_grad || !moments)))
     return fail(ctx, GSR_E_INVALID, "NULL argument");
-  for (const void* p : {static_cast<const void*>(sh), static_cast<const void*>(moments)})
-    if (reinterpret_cast<uintptr_t>(p) % 16) return fail(ctx, GSR_E_INVALID, "sh and moments must be 16-byte aligned");
+  for (const void* p : {static_cast<const void*>(gs), static_cast<const void*>(sh), static_cast<const void*>(sh_grad),
+                        static_cast<const void*>(moments)})
+    if (reinterpret_cast<uintptr_t>(p) % 16)
+      return fail(ctx, GSR_E_INVALID, "gs, sh, sh_grad and moments must be 16-byte aligned");
   launch_adam_sh(ctx->stream, *step, lr_sh_rest, gs, sh, grads, shape, sh_grad, moments);
   GSR_LAUNCHED(ctx);
   return GSR_OK;

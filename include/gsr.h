@@ -158,7 +158,7 @@ int gsr_adam(gsr_ctx* ctx, const gsr_step* step, float* gs, const float* grads, 
  * floats, zero at the start: SH m [n][48] at 0 and v at 48 n; parameter k of
  * mean 0-2, logScale 3-5, rotation 6-9, opacity 10 has m at 96 n + k n + i and
  * v at 107 n + k n + i) are updated in place; sh_grad as written by gsr_shape /
- * gsr_batch_step.  sh, sh_grad and moments are 16-byte aligned. */
+ * gsr_batch_step.  gs, sh, sh_grad and moments are 16-byte aligned. */
 int gsr_adam_sh(gsr_ctx* ctx, const gsr_step* step, float lr_sh_rest, float* gs, float* sh, const float* grads,
                 const float* shape, const float* sh_grad, float* moments);
 

@@ -468,3 +468,34 @@ def test_adam_sh_matches_restatement(ctx):
     np.testing.assert_allclose(m_got, m_ref, rtol=1e-6, atol=1e-9)
     for b in bufs:
         b.free()
+
+
+def test_extension_entry_points_reject_bad_arguments(ctx):
+    """The SSIM window, lambda and the SH3 Adam's alignment are checked before
+    any launch; each failure raises with the library's message."""
+    W, H = 24, 16
+    image, target = lcg_images(W, H)
+    ib, tb = ctx.upload(image.astype(np.float32)), ctx.upload(target.astype(np.float32))
+    gb = ctx.alloc(image.size * 4)
+    cam = scenes.Camera(view=np.eye(4)[:3].ravel(), fx=1.0, fy=1.0, cx=0.0, cy=0.0, width=W, height=H)
+    for window, lam in ((9, 0.2), (11, 1.5), (7, -0.1)):
+        with pytest.raises(lib.GsrError) as ei:
+            ctx.loss_grad(cam, ib, tb, lam, gb, window=window)
+        assert ei.value.code == lib.GSR_E_INVALID
+    n = 64
+    bufs = [ctx.alloc(n * b) for b in (64, 192, 64, 40, 192, 480 + 16)]
+    lr = {"color": 5e-3, "opacity": 0.05, "mean": 1.6e-4, "scale": 5e-3, "rotation": 1e-3, "sh_rest": 2.5e-4}
+    shifted = lib.Buffer(ctx, bufs[5].ptr + 4, n * 480)  # a view 4 bytes into the moments
+    with pytest.raises(lib.GsrError) as ei:
+        ctx.adam_sh(1.0, n, lr, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], shifted)
+    assert "16-byte aligned" in str(ei.value)
+    ctx.adam_sh(1.0, n, lr, *bufs[:5], bufs[5])  # aligned: accepted
+    ctx.sync()
+    sc = scenes.config(0, n=500)
+    batch = lib.Batch(ctx, sc.cams, sc.n, 0)
+    with pytest.raises(lib.GsrError):
+        batch.set_ssim_window(9)
+    batch.set_ssim_window(lib.SSIM_GAUSS11)
+    batch.close()
+    for b in [ib, tb, gb] + bufs:
+        b.free()
