@@ -63,9 +63,9 @@ struct gsr_batch {
   uint8_t* band_open = nullptr; // per tile, 4 bytes: band still open after the last forward pass
   int ssim_window = GSR_SSIM_BOX7;
   uint32_t* walk = nullptr;     // [kMaxPasses][total_tiles]: entries the forward examined per pass
-  TileWork* heavy = nullptr;    // [4 total_tiles]: band items of the split tiles of a launch
+  TileWork* heavy = nullptr;    // [2 total_tiles]: half-tile items of the split tiles of a launch
   uint8_t* split = nullptr;     // [total_tiles]: tile split in the current launch
-  // Tiles are blended one band per wave when their cost exceeds these (0: never):
+  // Tiles are blended as two half-tile items when their cost exceeds these (0: never):
   // the forward of the first depth slice (most tiles saturate early, so list
   // length says little about the walk), the forward of later slices (unsaturated
   // tiles walk their whole segment) and the backward (entries the forward walked).
@@ -156,7 +156,7 @@ int alloc_all(gsr_batch* b) {
       (rc = dalloc(b, &b->tstate, b->total_pixels)) || (rc = dalloc(b, &b->done, b->total_tiles)) ||
       (rc = dalloc(b, &b->band_open, size_t(b->total_tiles) * 4)) ||
       (rc = dalloc(b, &b->walk, size_t(kMaxPasses) * b->total_tiles)) ||
-      (rc = dalloc(b, &b->heavy, size_t(b->total_tiles) * 4)) || (rc = dalloc(b, &b->split, b->total_tiles)) ||
+      (rc = dalloc(b, &b->heavy, size_t(b->total_tiles) * 2)) || (rc = dalloc(b, &b->split, b->total_tiles)) ||
       (rc = dalloc(b, &b->flags, 64)) || (rc = dalloc(b, &b->image, b->total_pixels * 3)) ||
       (rc = dalloc(b, &b->dl, b->total_pixels * 3)) ||
       (rc = dalloc(b, &b->vgrads, size_t(b->V) * std::max<uint32_t>(b->n, 1) * 16)))
@@ -631,11 +631,11 @@ int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const flo
   }
   int rc = gsr_batch_configure(b, best_tb, best_ch);
   if (rc) return rc;
-  // Then the walk above which the backward blends a tile one band per wave: low
-  // enough to shorten the launch's tail, high enough not to repeat each entry's
-  // staging four times over for tiles that are merely long (scenes of dense
-  // clusters prefer a higher one).
-  const uint32_t splits[] = {1024, 2048, 4096};
+  // Then the walk above which the backward blends a tile as two half-tile items:
+  // low enough to shorten the launch's tail, high enough not to repeat each
+  // entry's staging for tiles that are merely long (scenes of dense clusters
+  // prefer a higher one).
+  const uint32_t splits[] = {512, 1024, 2048, 4096};
   double best_split_t = 1e30;
   uint32_t best_split = b->split_backward;
   for (uint32_t sp : splits) {

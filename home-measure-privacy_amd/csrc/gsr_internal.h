@@ -125,12 +125,13 @@ struct RasterArgs {
   uint64_t grads_view_stride;  // elements between views' gradient records (0: shared)
   uint32_t* queue;       // persistent mode: work counter (zeroed before launch)
   uint32_t tile_batch;   // tiles claimed per queue pop
-  // Band splitting (batched step), on when split != null: a tile with split[g]
-  // set is blended by four waves, one band each.  Items [0, heavy_count[0]) are
-  // those band items, then come the num_work tiles of `work` in order, the split
-  // ones returning at once.  Both are written by launch_split_list.
+  // Tile splitting (batched step), on when split != null: a tile with split[g]
+  // set is blended by two waves, one per half (bands 0-1, 2-3).  Items
+  // [0, heavy_count[0]) are those half-tile items, then come the num_work tiles
+  // of `work` in order, the split ones returning at once.  Both are written by
+  // launch_split_list.
   const uint8_t* split;          // [num_work]
-  const TileWork* heavy;         // [4 num_work]
+  const TileWork* heavy;         // [2 num_work]
   const uint32_t* heavy_count;
   // Forward over passes: pass blends segment `pass`, resuming from tstate (per
   // pixel transmittance) and image, and skips tiles flagged done (saturated by
@@ -147,7 +148,7 @@ struct RasterArgs {
 // blends every tile), else done[g] |= (no band open).
 void launch_done_update(hipStream_t s, uint32_t total_tiles, const uint8_t* band_open, uint8_t* done, int assign);
 // Marks the tiles of a blend launch whose cost exceeds split_cost and lists
-// their band items (heavy_count zeroed before).  Cost of tile g: with walk (the
+// their half-tile items (heavy_count zeroed before).  Cost of tile g: with walk (the
 // backward), the entries the forward examined summed over num_passes
 // (walk[p][g]); otherwise its entries over ranges[0 .. num_passes).
 struct SplitArgs {

@@ -190,9 +190,9 @@ int gsr_batch_entries(gsr_batch* batch, uint64_t* entries);
 /* Tuning knobs: tiles claimed per queue pop (0 = keep) and Gaussians per
  * binning chunk (0 = keep).  gsr_batch_autotune times candidates on the given
  * inputs and keeps the fastest; it writes the choice to *tile_batch / *chunk.
- * It also picks the walk length above which the backward blends a tile one
- * 4-row band per wave (1024, 2048 or 4096 entries).  Results do not depend on
- * any of these. */
+ * It also picks the walk length above which the backward blends a tile as
+ * two half-tile items (512 to 4096 entries).  Results do not depend on any of
+ * these. */
 int gsr_batch_configure(gsr_batch* batch, uint32_t tile_batch, uint32_t chunk);
 /* SSIM window of the batched step's loss (default GSR_SSIM_BOX7). */
 int gsr_batch_set_ssim_window(gsr_batch* batch, int window);
