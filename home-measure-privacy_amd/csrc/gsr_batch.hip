@@ -29,6 +29,9 @@ int ctx_num_cus(gsr_ctx* ctx);
 int ctx_fail(gsr_ctx* ctx, int code, const char* msg);
 }  // namespace gsr
 
+constexpr uint32_t kQueueBase = 32;
+constexpr uint32_t kFlagWords = kQueueBase + 9 * (gsr::kMaxPasses + 1);
+
 struct gsr_batch {
   gsr_ctx* ctx = nullptr;
   uint32_t V = 0, n = 0;
@@ -72,7 +75,10 @@ struct gsr_batch {
   uint32_t split_first = 0, split_later = 0, split_backward = 1024;
   uint32_t* order = nullptr;
   uint64_t order_cap = 0;
-  uint32_t* flags = nullptr;  // [0] overflow, [1 + p] forward queue of pass p, [1 + kMaxPasses] backward queue
+  // [0] overflow, [16 + p] split-item count of forward pass p, [16 + kMaxPasses]
+  // the backward's; from kQueueBase, 9 work counters per forward pass, then the
+  // backward's (a shared one and one per die part).
+  uint32_t* flags = nullptr;
   float* image = nullptr;
   float* dl = nullptr;
   float* coef = nullptr;
@@ -157,7 +163,7 @@ int alloc_all(gsr_batch* b) {
       (rc = dalloc(b, &b->band_open, size_t(b->total_tiles) * 4)) ||
       (rc = dalloc(b, &b->walk, size_t(kMaxPasses) * b->total_tiles)) ||
       (rc = dalloc(b, &b->heavy, size_t(b->total_tiles) * 2)) || (rc = dalloc(b, &b->split, b->total_tiles)) ||
-      (rc = dalloc(b, &b->flags, 64)) || (rc = dalloc(b, &b->image, b->total_pixels * 3)) ||
+      (rc = dalloc(b, &b->flags, kFlagWords)) || (rc = dalloc(b, &b->image, b->total_pixels * 3)) ||
       (rc = dalloc(b, &b->dl, b->total_pixels * 3)) ||
       (rc = dalloc(b, &b->vgrads, size_t(b->V) * std::max<uint32_t>(b->n, 1) * 16)))
     return rc;
@@ -215,7 +221,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   auto mark = [&]() {
     if (prof) hipEventRecord(prof->ev[stage++], s);
   };
-  launch_zero(s, b->flags, 64 * sizeof(uint32_t));
+  launch_zero(s, b->flags, kFlagWords * sizeof(uint32_t));
   launch_zero(s, b->walk, size_t(b->num_passes) * b->total_tiles * sizeof(uint32_t));
   if (prof && prof->last) launch_zero(s, prof->last, size_t(b->num_passes) * b->total_tiles * sizeof(uint32_t));
   mark();
@@ -269,7 +275,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
                     b->flags);
     if (prof) hipEventRecord(e2, s);
     a.pass = int(p);
-    a.queue = b->flags + 1 + p;
+    a.queue = b->flags + kQueueBase + 9 * p;
     const uint32_t split_cost = p ? b->split_later : b->split_first;
     a.split = split_cost ? b->split : nullptr;
     a.heavy = b->heavy;
@@ -328,7 +334,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
     sa.heavy_count = b->flags + 16 + kMaxPasses;
     launch_split_list(s, sa);
   }
-  a.queue = b->flags + 1 + kMaxPasses;
+  a.queue = b->flags + kQueueBase + 9 * kMaxPasses;
   launch_backward(s, a, 1, b->grid_bwd);
   mark();
   launch_shape(s, b->d_cams, b->V, gs, sh, b->deg, b->n, b->vgrads, uint64_t(b->n) * 16, shape, sh_grad, grads, 1,

@@ -123,7 +123,7 @@ struct RasterArgs {
   const float* dl;       // backward: dLoss/dPixel, concatenated views
   float* grads;          // backward: [V][n][16]
   uint64_t grads_view_stride;  // elements between views' gradient records (0: shared)
-  uint32_t* queue;       // persistent mode: work counter (zeroed before launch)
+  uint32_t* queue;       // persistent mode: 9 work counters (zeroed before launch)
   uint32_t tile_batch;   // tiles claimed per queue pop
   // Tile splitting (batched step), on when split != null: a tile with split[g]
   // set is blended by two waves, one per half (bands 0-1, 2-3).  Items

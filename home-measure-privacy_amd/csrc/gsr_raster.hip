@@ -39,11 +39,6 @@ struct Tile {
   int first, last;      // bands of the tile this item covers
 };
 
-// Items of a launch: the half-tile items of split tiles, then one per tile.
-__device__ __forceinline__ uint32_t num_items(const RasterArgs& a) {
-  return (a.split ? a.heavy_count[0] : 0u) + a.num_work;
-}
-
 // The tile (and bands) of item w; false for the whole-tile items of split tiles.
 __device__ __forceinline__ bool load_tile(const RasterArgs& a, uint32_t w, Tile& t) {
   const uint32_t nh = a.split ? a.heavy_count[0] : 0u;
@@ -383,16 +378,42 @@ __device__ inline uint32_t pop_work(uint32_t* queue, uint32_t n) {
   return __builtin_amdgcn_readfirstlane(w0);
 }
 
+// Work distribution of the persistent kernels.  Workgroups are dealt to the
+// chip's kXcds dies in turn (workgroup i on die i mod kXcds), each die with its
+// own L2, and neighbouring tiles of a view share most of their splat records.
+// So the whole tiles are cut into kXcds contiguous parts, part d served first to
+// the workgroups of die d from its own counter; a die that runs out helps the
+// others in turn.  The split tiles' items come first, from a shared counter.
+// queue: [0] the split items' counter, [1 + d] part d's.
+constexpr uint32_t kXcds = 8;
+
+template <class Blend>
+__device__ __forceinline__ void for_each_item(const RasterArgs& a, Blend&& blend) {
+  const uint32_t nh = a.split ? a.heavy_count[0] : 0u;
+  for (;;) {
+    const uint32_t w0 = pop_work(a.queue, a.tile_batch);
+    if (w0 >= nh) break;
+    const uint32_t w1 = min(nh, w0 + a.tile_batch);
+    for (uint32_t w = w0; w < w1; ++w) blend(w);
+  }
+  const uint32_t home = blockIdx.x % kXcds;
+  for (uint32_t k = 0; k < kXcds; ++k) {
+    const uint32_t part = (home + k) % kXcds;
+    const uint32_t lo = nh + uint32_t(uint64_t(a.num_work) * part / kXcds);
+    const uint32_t hi = nh + uint32_t(uint64_t(a.num_work) * (part + 1) / kXcds);
+    for (;;) {
+      const uint32_t w0 = lo + pop_work(a.queue + 1 + part, a.tile_batch);
+      if (w0 >= hi) break;
+      const uint32_t w1 = min(hi, w0 + a.tile_batch);
+      for (uint32_t w = w0; w < w1; ++w) blend(w);
+    }
+  }
+}
+
 template <bool kLast>
 __global__ __launch_bounds__(64) void forward_persistent_kernel(RasterArgs a) {
   __shared__ float4 lds[64][3];
-  const uint32_t items = num_items(a);
-  for (;;) {
-    uint32_t w0 = pop_work(a.queue, a.tile_batch);
-    if (w0 >= items) break;
-    uint32_t w1 = min(items, w0 + a.tile_batch);
-    for (uint32_t w = w0; w < w1; ++w) forward_tile<kLast>(a, w, lds);
-  }
+  for_each_item(a, [&](uint32_t w) { forward_tile<kLast>(a, w, lds); });
 }
 
 // ---- backward ------------------------------------------------------------------
@@ -566,13 +587,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ba
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void backward_persistent_kernel(RasterArgs a) {
   __shared__ float4 lds[64][4];
   __shared__ uint32_t gi_lds[64];
-  const uint32_t items = num_items(a);
-  for (;;) {
-    uint32_t w0 = pop_work(a.queue, a.tile_batch);
-    if (w0 >= items) break;
-    uint32_t w1 = min(items, w0 + a.tile_batch);
-    for (uint32_t w = w0; w < w1; ++w) backward_tile(a, w, lds, gi_lds);
-  }
+  for_each_item(a, [&](uint32_t w) { backward_tile(a, w, lds, gi_lds); });
 }
 
 // ---- loss gradient (gpu.js:336-409) ---------------------------------------------
