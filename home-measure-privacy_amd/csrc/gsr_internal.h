@@ -126,11 +126,11 @@ struct RasterArgs {
   uint32_t* queue;       // persistent mode: 9 work counters (zeroed before launch)
   uint32_t tile_batch;   // tiles claimed per queue pop
   // Tile splitting (batched step), on when split != null: a tile with split[g]
-  // set is blended by two waves, one per half (bands 0-1, 2-3).  Items
-  // [0, heavy_count[0]) are those half-tile items, then come the num_work tiles
-  // of `work` in order, the split ones returning at once.  Both are written by
-  // launch_split_list.
-  const uint8_t* split;          // [num_work]
+  // set (g: global tile id) is blended by two waves, one per half (bands 0-1,
+  // 2-3).  Items [0, heavy_count[0]) are those half-tile items, then come the
+  // num_work tiles of `work` in order (any order of the tiles), the split ones
+  // returning at once.  Both are written by launch_split_list.
+  const uint8_t* split;          // [total tiles]
   const TileWork* heavy;         // [2 num_work]
   const uint32_t* heavy_count;
   // Forward over passes: pass blends segment `pass`, resuming from tstate (per

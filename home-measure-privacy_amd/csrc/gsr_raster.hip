@@ -42,14 +42,8 @@ struct Tile {
 // The tile (and bands) of item w; false for the whole-tile items of split tiles.
 __device__ __forceinline__ bool load_tile(const RasterArgs& a, uint32_t w, Tile& t) {
   const uint32_t nh = a.split ? a.heavy_count[0] : 0u;
-  TileWork tw;
-  if (w < nh) {
-    tw = a.heavy[w];
-  } else {
-    const uint32_t g = w - nh;
-    if (a.split && a.split[g]) return false;
-    tw = a.work[g];
-  }
+  const bool whole = w >= nh;
+  const TileWork tw = whole ? a.work[w - nh] : a.heavy[w];
   const uint32_t first = work_first(tw), last = work_last(tw);
   const uint32_t view = work_view(tw);
   ViewDesc vd = a.views[view];
@@ -62,6 +56,8 @@ __device__ __forceinline__ bool load_tile(const RasterArgs& a, uint32_t w, Tile&
   t.width = vd.width;
   t.height = vd.height;
   t.gtile = vd.tile_base + ty * vd.tiles_x + tx;
+  // a split tile is blended by its half-tile items only (split is indexed by tile id)
+  if (whole && a.split && a.split[t.gtile]) return false;
   t.start = t.end = 0;
   t.order = a.order;
   t.splats = a.splats + vd.splat_base * 12;
