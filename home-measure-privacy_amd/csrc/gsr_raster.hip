@@ -409,7 +409,7 @@ __device__ __forceinline__ void for_each_item(const RasterArgs& a, Blend&& blend
 template <bool kLast>
 __global__ __launch_bounds__(64) void forward_persistent_kernel(RasterArgs a) {
   __shared__ float4 lds[64][3];
-  for_each_item(a, [&](uint32_t w) { forward_tile<kLast>(a, w, lds); });
+  for_each_item(a, [&](uint32_t w) __attribute__((always_inline)) { forward_tile<kLast>(a, w, lds); });
 }
 
 // ---- backward ------------------------------------------------------------------
@@ -583,7 +583,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ba
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void backward_persistent_kernel(RasterArgs a) {
   __shared__ float4 lds[64][4];
   __shared__ uint32_t gi_lds[64];
-  for_each_item(a, [&](uint32_t w) { backward_tile(a, w, lds, gi_lds); });
+  for_each_item(a, [&](uint32_t w) __attribute__((always_inline)) { backward_tile(a, w, lds, gi_lds); });
 }
 
 // ---- loss gradient (gpu.js:336-409) ---------------------------------------------
