@@ -608,6 +608,16 @@ int gsr_batch_profile(gsr_batch* b, const float* gs, const float* sh, const floa
   return GSR_OK;
 }
 
+int gsr_batch_set_split(gsr_batch* b, uint32_t forward_later, uint32_t backward) {
+  if (!b) return GSR_E_INVALID;
+  if (forward_later != b->split_later || backward != b->split_backward) {
+    b->split_later = forward_later;
+    b->split_backward = backward;
+    drop_graph(b);  // the thresholds are launch arguments of the captured step
+  }
+  return GSR_OK;
+}
+
 int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const float* targets, float lambda,
                        float* grads, float* shape, float* sh_grad, uint32_t* tile_batch, uint32_t* chunk) {
   if (!b) return GSR_E_INVALID;
@@ -661,6 +671,30 @@ int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const flo
     }
   }
   b->split_backward = best_split;
+  drop_graph(b);
+  // And whether (and from which list length) the forward's later slices split
+  // their tiles: their unsaturated tiles walk whole segments, so a few long
+  // lists can end the launch late; splitting them pays on some scenes only.
+  const uint32_t later[] = {0, 1024, 2048, 4096};
+  double best_later_t = 1e30;
+  uint32_t best_later = b->split_later;
+  for (uint32_t sl : later) {
+    b->split_later = sl;
+    drop_graph(b);
+    rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
+    if (rc) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < 3; ++k) {
+      rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
+      if (rc) return rc;
+    }
+    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (dt < best_later_t) {
+      best_later_t = dt;
+      best_later = sl;
+    }
+  }
+  b->split_later = best_later;
   drop_graph(b);
   if (tile_batch) *tile_batch = best_tb;
   if (chunk) *chunk = best_ch;

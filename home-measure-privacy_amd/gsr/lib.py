@@ -24,7 +24,7 @@ EXPORTS = (
     "gsr_clear", "gsr_copy", "gsr_synchronize", "gsr_project", "gsr_bin_sort", "gsr_forward", "gsr_loss_grad", "gsr_backward",
     "gsr_shape", "gsr_adam", "gsr_adam_sh", "gsr_stats", "gsr_batch_create", "gsr_batch_destroy", "gsr_batch_set_cameras",
     "gsr_batch_step", "gsr_batch_entries", "gsr_batch_configure", "gsr_batch_autotune", "gsr_batch_profile",
-    "gsr_batch_set_passes", "gsr_loss_grad_window", "gsr_batch_set_ssim_window",
+    "gsr_batch_set_passes", "gsr_loss_grad_window", "gsr_batch_set_ssim_window", "gsr_batch_set_split",
 )
 SSIM_BOX7, SSIM_GAUSS11 = 7, 11
 STAGES = ("project", "depth_sort", "bin_count", "bin_emit", "forward", "loss_grad", "backward", "chain")
@@ -93,6 +93,7 @@ def load():
         "gsr_batch_set_passes": (i32, [P, u32, u32]),
         "gsr_loss_grad_window": (i32, [P, ctypes.POINTER(Camera), P, P, f32, i32, P]),
         "gsr_batch_set_ssim_window": (i32, [P, i32]),
+        "gsr_batch_set_split": (i32, [P, u32, u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -281,6 +282,10 @@ class Batch:
 
     def set_passes(self, passes: int, growth: int = 2):
         self.ctx._check(self.lib.gsr_batch_set_passes(self.h, passes, growth))
+
+    def set_split(self, forward_later: int, backward: int):
+        """Split thresholds (0 = never) of the forward's later slices and the backward."""
+        self.ctx._check(self.lib.gsr_batch_set_split(self.h, forward_later, backward))
 
     def set_ssim_window(self, window: int):
         self.ctx._check(self.lib.gsr_batch_set_ssim_window(self.h, window))

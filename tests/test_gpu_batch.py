@@ -30,7 +30,7 @@ def small_batch_scene(n=20000, views=5, seed=0):
     return sc
 
 
-def run_batch(ctx, sc, graph=True, tile_batch=0, chunk=0, steps=1, images=False, passes=None, growth=2):
+def run_batch(ctx, sc, graph=True, tile_batch=0, chunk=0, steps=1, images=False, passes=None, growth=2, split=None):
     n = sc.n
     gsb, shb = ctx.upload(sc.gs), ctx.upload(sc.sh)
     tgt = ctx.upload(np.concatenate([t.ravel() for t in sc.targets]).astype(np.float32))
@@ -41,6 +41,8 @@ def run_batch(ctx, sc, graph=True, tile_batch=0, chunk=0, steps=1, images=False,
     b.configure(tile_batch, chunk)
     if passes is not None:
         b.set_passes(passes, growth)
+    if split is not None:
+        b.set_split(*split)
     outs = []
     for _ in range(steps):
         b.step(gsb, shb, tgt, 0.0, grads, shape, shg, img, graph)
@@ -120,6 +122,24 @@ def test_depth_slice_passes_match_full_lists(ctx, passes, growth):
         assert rel_l2(sh, full[0][2]) < 1e-5
     assert e_sl.sum() < e_full.sum()
     log(f"  entries full {int(e_full.sum())} sliced {int(e_sl.sum())}")
+
+
+@pytest.mark.parametrize("split", [(64, 64), (0, 0), (200, 0)])
+def test_tile_splitting_leaves_results_unchanged(ctx, split):
+    """Blending long tiles as two half-tile items (in the forward's later depth
+    slices and in the backward) renders the same images bit for bit, with the
+    same gradients up to float atomic summation order.  Low thresholds split
+    most tiles."""
+    sc = small_batch_scene(n=40000, views=4, seed=7)
+    sc.gs[:, 7] += 1.0
+    ref, _ = run_batch(ctx, sc, graph=False, images=True, passes=3, growth=4, split=(0, 0))
+    log(f"split forward/backward {split}")
+    got, _ = run_batch(ctx, sc, graph=True, images=True, passes=3, growth=4, split=split, steps=2)
+    for g, s, sh, img in got:
+        np.testing.assert_array_equal(img, ref[0][3])
+        assert rel_l2(g[:, :4], ref[0][0][:, :4]) < 1e-5
+        assert rel_l2(s, ref[0][1]) < 1e-5
+        assert rel_l2(sh, ref[0][2]) < 1e-5
 
 
 def test_batch_matches_fp64_oracle(ctx):
