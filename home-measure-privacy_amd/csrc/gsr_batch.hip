@@ -66,6 +66,7 @@ struct gsr_batch {
   uint8_t* band_open = nullptr; // per tile, 4 bytes: band still open after the last forward pass
   int ssim_window = GSR_SSIM_BOX7;
   uint32_t* walk = nullptr;     // [kMaxPasses][total_tiles]: entries the forward examined per pass
+  uint32_t* prev_walk = nullptr;  // [total_tiles]: the previous step's first-slice walk (a cost estimate)
   TileWork* heavy = nullptr;    // [2 total_tiles]: half-tile items of the split tiles of a launch
   uint8_t* split = nullptr;     // [total_tiles]: tile split in the current launch
   // Tiles are blended as two half-tile items when their cost exceeds these (0: never):
@@ -162,6 +163,7 @@ int alloc_all(gsr_batch* b) {
       (rc = dalloc(b, &b->tstate, b->total_pixels)) || (rc = dalloc(b, &b->done, b->total_tiles)) ||
       (rc = dalloc(b, &b->band_open, size_t(b->total_tiles) * 4)) ||
       (rc = dalloc(b, &b->walk, size_t(kMaxPasses) * b->total_tiles)) ||
+      (rc = dalloc(b, &b->prev_walk, b->total_tiles)) ||
       (rc = dalloc(b, &b->heavy, size_t(b->total_tiles) * 2)) || (rc = dalloc(b, &b->split, b->total_tiles)) ||
       (rc = dalloc(b, &b->flags, kFlagWords)) || (rc = dalloc(b, &b->image, b->total_pixels * 3)) ||
       (rc = dalloc(b, &b->dl, b->total_pixels * 3)) ||
@@ -222,6 +224,9 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
     if (prof) hipEventRecord(prof->ev[stage++], s);
   };
   launch_zero(s, b->flags, kFlagWords * sizeof(uint32_t));
+  // The first slice's walk of the previous step estimates this step's: the
+  // forward splits its longest tiles by it (scheduling only).
+  if (b->split_first) launch_copy(s, b->prev_walk, b->walk, size_t(b->total_tiles) * sizeof(uint32_t));
   launch_zero(s, b->walk, size_t(b->num_passes) * b->total_tiles * sizeof(uint32_t));
   if (prof && prof->last) launch_zero(s, prof->last, size_t(b->num_passes) * b->total_tiles * sizeof(uint32_t));
   mark();
@@ -287,6 +292,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
       sa.split_cost = split_cost;
       sa.ranges[0] = a.ranges[p];
       sa.num_passes = 1;
+      if (p == 0) sa.walk = b->prev_walk;
       sa.split = b->split;
       sa.heavy = b->heavy;
       sa.heavy_count = b->flags + 16 + p;
@@ -453,7 +459,7 @@ void gsr_batch_destroy(gsr_batch* b) {
   void* ptrs[] = {b->d_cams, b->d_views, b->d_work, b->splats, b->keys, b->keys_alt, b->vals, b->vals_alt,
                   b->rects, b->rect_sorted, b->hist, b->piece, b->matrix, b->tile_tot, b->entries, b->ebase,
                   b->run, b->ranges, b->tstate, b->done, b->order, b->flags, b->image, b->dl, b->coef, b->vgrads,
-                  b->band_open, b->walk, b->heavy, b->split};
+                  b->band_open, b->walk, b->heavy, b->split, b->prev_walk};
   for (void* p : ptrs) dfree(p);
   if (b->pinned) hipHostFree(b->pinned);
   delete b;
@@ -608,9 +614,10 @@ int gsr_batch_profile(gsr_batch* b, const float* gs, const float* sh, const floa
   return GSR_OK;
 }
 
-int gsr_batch_set_split(gsr_batch* b, uint32_t forward_later, uint32_t backward) {
+int gsr_batch_set_split(gsr_batch* b, uint32_t forward_first, uint32_t forward_later, uint32_t backward) {
   if (!b) return GSR_E_INVALID;
-  if (forward_later != b->split_later || backward != b->split_backward) {
+  if (forward_first != b->split_first || forward_later != b->split_later || backward != b->split_backward) {
+    b->split_first = forward_first;
     b->split_later = forward_later;
     b->split_backward = backward;
     drop_graph(b);  // the thresholds are launch arguments of the captured step
@@ -695,6 +702,31 @@ int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const flo
     }
   }
   b->split_later = best_later;
+  drop_graph(b);
+  // The first slice by the previous step's walk (its tiles mostly saturate early,
+  // so their list lengths say little).
+  const uint32_t firsts[] = {0, 1024, 2048};
+  double best_first_t = 1e30;
+  uint32_t best_first = b->split_first;
+  for (uint32_t sf : firsts) {
+    b->split_first = sf;
+    drop_graph(b);
+    for (int k = 0; k < 2; ++k) {  // the second fills the walk estimate
+      rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
+      if (rc) return rc;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < 3; ++k) {
+      rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
+      if (rc) return rc;
+    }
+    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (dt < best_first_t) {
+      best_first_t = dt;
+      best_first = sf;
+    }
+  }
+  b->split_first = best_first;
   drop_graph(b);
   if (tile_batch) *tile_batch = best_tb;
   if (chunk) *chunk = best_ch;

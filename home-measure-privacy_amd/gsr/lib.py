@@ -93,7 +93,7 @@ def load():
         "gsr_batch_set_passes": (i32, [P, u32, u32]),
         "gsr_loss_grad_window": (i32, [P, ctypes.POINTER(Camera), P, P, f32, i32, P]),
         "gsr_batch_set_ssim_window": (i32, [P, i32]),
-        "gsr_batch_set_split": (i32, [P, u32, u32]),
+        "gsr_batch_set_split": (i32, [P, u32, u32, u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -283,9 +283,10 @@ class Batch:
     def set_passes(self, passes: int, growth: int = 2):
         self.ctx._check(self.lib.gsr_batch_set_passes(self.h, passes, growth))
 
-    def set_split(self, forward_later: int, backward: int):
-        """Split thresholds (0 = never) of the forward's later slices and the backward."""
-        self.ctx._check(self.lib.gsr_batch_set_split(self.h, forward_later, backward))
+    def set_split(self, forward_first: int, forward_later: int, backward: int):
+        """Split thresholds (0 = never): the forward's first slice (by the previous
+        step's walk), its later slices (by list length) and the backward."""
+        self.ctx._check(self.lib.gsr_batch_set_split(self.h, forward_first, forward_later, backward))
 
     def set_ssim_window(self, window: int):
         self.ctx._check(self.lib.gsr_batch_set_ssim_window(self.h, window))

@@ -194,10 +194,11 @@ int gsr_batch_entries(gsr_batch* batch, uint64_t* entries);
  * depend on any of these. */
 int gsr_batch_configure(gsr_batch* batch, uint32_t tile_batch, uint32_t chunk);
 /* Walk lengths above which a tile is blended as two half-tile items: in the
- * forward's later depth slices (by list length; 0 = never, the default) and in
- * the backward (by the entries the forward walked; 0 = never, default 1024).
- * gsr_batch_autotune picks both.  Results do not depend on them. */
-int gsr_batch_set_split(gsr_batch* batch, uint32_t forward_later, uint32_t backward);
+ * forward's first depth slice (by the previous step's walk of the tile), in its
+ * later slices (by list length) and in the backward (by the entries the forward
+ * walked).  0 = never; defaults 0, 0 and 1024.  gsr_batch_autotune picks all
+ * three.  Results do not depend on them. */
+int gsr_batch_set_split(gsr_batch* batch, uint32_t forward_first, uint32_t forward_later, uint32_t backward);
 /* SSIM window of the batched step's loss (default GSR_SSIM_BOX7). */
 int gsr_batch_set_ssim_window(gsr_batch* batch, int window);
 /* Depth-slice passes of the batched step (1 to 8; default 2 with growth 8 for

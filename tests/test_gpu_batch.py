@@ -124,15 +124,16 @@ def test_depth_slice_passes_match_full_lists(ctx, passes, growth):
     log(f"  entries full {int(e_full.sum())} sliced {int(e_sl.sum())}")
 
 
-@pytest.mark.parametrize("split", [(64, 64), (0, 0), (200, 0)])
+@pytest.mark.parametrize("split", [(64, 64, 64), (0, 0, 0), (0, 200, 0), (100, 0, 0)])
 def test_tile_splitting_leaves_results_unchanged(ctx, split):
-    """Blending long tiles as two half-tile items (in the forward's later depth
-    slices and in the backward) renders the same images bit for bit, with the
-    same gradients up to float atomic summation order.  Low thresholds split
-    most tiles."""
+    """Blending long tiles as two half-tile items (in the forward's first depth
+    slice by the previous step's walk, its later slices and the backward)
+    renders the same images bit for bit, with the same gradients up to float
+    atomic summation order.  Low thresholds split most tiles; the second step
+    splits the first slice by the first step's walk."""
     sc = small_batch_scene(n=40000, views=4, seed=7)
     sc.gs[:, 7] += 1.0
-    ref, _ = run_batch(ctx, sc, graph=False, images=True, passes=3, growth=4, split=(0, 0))
+    ref, _ = run_batch(ctx, sc, graph=False, images=True, passes=3, growth=4, split=(0, 0, 0))
     log(f"split forward/backward {split}")
     got, _ = run_batch(ctx, sc, graph=True, images=True, passes=3, growth=4, split=split, steps=2)
     for g, s, sh, img in got:
