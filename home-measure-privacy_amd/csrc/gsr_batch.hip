@@ -30,6 +30,9 @@ int ctx_fail(gsr_ctx* ctx, int code, const char* msg);
 }  // namespace gsr
 
 constexpr uint32_t kQueueBase = 32;
+// Per-view gradient records hold the nine screen-space terms at 0-8, packed 12
+// floats apart (48 bytes: wave reads are contiguous, no padding fetched).
+constexpr uint32_t kRecord = 12;
 constexpr uint32_t kFlagWords = kQueueBase + 9 * (gsr::kMaxPasses + 1);
 
 struct gsr_batch {
@@ -167,7 +170,7 @@ int alloc_all(gsr_batch* b) {
       (rc = dalloc(b, &b->heavy, size_t(b->total_tiles) * 2)) || (rc = dalloc(b, &b->split, b->total_tiles)) ||
       (rc = dalloc(b, &b->flags, kFlagWords)) || (rc = dalloc(b, &b->image, b->total_pixels * 3)) ||
       (rc = dalloc(b, &b->dl, b->total_pixels * 3)) ||
-      (rc = dalloc(b, &b->vgrads, size_t(b->V) * std::max<uint32_t>(b->n, 1) * 16)))
+      (rc = dalloc(b, &b->vgrads, size_t(b->V) * std::max<uint32_t>(b->n, 1) * kRecord)))
     return rc;
   (void)s;
   return GSR_OK;
@@ -252,7 +255,8 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   a.state = nullptr;
   a.dl = b->dl;
   a.grads = b->vgrads;
-  a.grads_view_stride = uint64_t(b->n) * 16;
+  a.grads_view_stride = uint64_t(b->n) * kRecord;
+  a.grads_record = kRecord;
   a.tile_batch = b->tile_batch;
   a.tstate = b->num_passes > 1 ? b->tstate : nullptr;
   a.done = b->num_passes > 1 ? b->done : nullptr;
@@ -343,8 +347,8 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   a.queue = b->flags + kQueueBase + 9 * kMaxPasses;
   launch_backward(s, a, 1, b->grid_bwd);
   mark();
-  launch_shape(s, b->d_cams, b->V, gs, sh, b->deg, b->n, b->vgrads, uint64_t(b->n) * 16, shape, sh_grad, grads, 1,
-               prof && prof->last ? prof->touched : nullptr);
+  launch_shape(s, b->d_cams, b->V, gs, sh, b->deg, b->n, b->vgrads, uint64_t(b->n) * kRecord, shape, sh_grad, grads,
+               1, prof && prof->last ? prof->touched : nullptr, kRecord);
   mark();
 }
 

@@ -18,7 +18,7 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
                                                     int deg, uint32_t n, float* __restrict__ grads,
                                                     uint64_t stride, float* __restrict__ shape,
                                                     float* __restrict__ sh_grad, float* __restrict__ grads_out,
-                                                    int consume, uint32_t* __restrict__ touched) {
+                                                    int consume, uint32_t* __restrict__ touched, uint32_t record) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4* g4 = reinterpret_cast<const float4*>(gs) + size_t(i) * 4;
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
   float gcol[3] = {0.f, 0.f, 0.f};
   float gop = 0.0f;
   for (uint32_t v = 0; v < num_views; ++v) {
-    float4* r = reinterpret_cast<float4*>(grads + size_t(v) * stride + size_t(i) * 16);
+    float4* r = reinterpret_cast<float4*>(grads + size_t(v) * stride + size_t(i) * record);
     const float4 ga4 = r[0], gb4 = r[1], gc4 = r[2];
     const float gr[9] = {ga4.x, ga4.y, ga4.z, ga4.w, gb4.x, gb4.y, gb4.z, gb4.w, gc4.x};
     gcol[0] += gr[0]; gcol[1] += gr[1]; gcol[2] += gr[2];
@@ -336,10 +336,10 @@ __global__ __launch_bounds__(256) void stats_kernel(uint32_t n, const float* __r
 
 void launch_shape(hipStream_t s, const gsr_camera* cams, uint32_t num_views, const float* gs, const float* sh,
                   int sh_degree, uint32_t n, float* grads, uint64_t grads_view_stride, float* shape,
-                  float* sh_grad, float* grads_out, int consume, uint32_t* touched) {
+                  float* sh_grad, float* grads_out, int consume, uint32_t* touched, uint32_t record) {
   if (n == 0) return;
   hipLaunchKernelGGL(shape_kernel, dim3((n + 255) / 256), dim3(256), 0, s, cams, num_views, gs, sh, sh_degree, n,
-                     grads, grads_view_stride, shape, sh_grad, grads_out, consume, touched);
+                     grads, grads_view_stride, shape, sh_grad, grads_out, consume, touched, record);
 }
 
 void launch_adam(hipStream_t s, const gsr_step& step, float* gs, const float* grads, const float* shape,

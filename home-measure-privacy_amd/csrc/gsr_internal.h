@@ -123,6 +123,7 @@ struct RasterArgs {
   const float* dl;       // backward: dLoss/dPixel, concatenated views
   float* grads;          // backward: [V][n][16]
   uint64_t grads_view_stride;  // elements between views' gradient records (0: shared)
+  uint32_t grads_record;       // floats per gradient record (0: the reference's 16)
   uint32_t* queue;       // persistent mode: 9 work counters (zeroed before launch)
   uint32_t tile_batch;   // tiles claimed per queue pop
   // Tile splitting (batched step), on when split != null: a tile with split[g]
@@ -182,9 +183,11 @@ void launch_zero(hipStream_t s, void* p, size_t bytes);
 // consume != 0 every per-view record that held a gradient is zeroed after use,
 // which leaves the per-view accumulators clear for the next backward.
 // touched (optional, [num_views]) counts the Gaussians with any gradient per view.
+// record: floats per per-view gradient record (16, or 12 for the batch's packed
+// records; the nine terms sit at 0-8 either way).
 void launch_shape(hipStream_t s, const gsr_camera* cams, uint32_t num_views, const float* gs, const float* sh,
                   int sh_degree, uint32_t n, float* grads, uint64_t grads_view_stride, float* shape,
-                  float* sh_grad, float* grads_out, int consume, uint32_t* touched = nullptr);
+                  float* sh_grad, float* grads_out, int consume, uint32_t* touched = nullptr, uint32_t record = 16);
 void launch_adam_sh(hipStream_t s, const gsr_step& step, float lr_sh_rest, float* gs, float* sh, const float* grads,
                     const float* shape, const float* sh_grad, float* moments);
 void launch_adam(hipStream_t s, const gsr_step& step, float* gs, const float* grads, const float* shape,

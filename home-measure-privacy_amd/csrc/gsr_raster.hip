@@ -495,6 +495,7 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
   const float band0 = float(t.y0) + 0.5f;
   const uint32_t items = item_bands(t);
   float* grads = a.grads + size_t(t.view) * a.grads_view_stride;
+  const uint32_t record = a.grads_record ? a.grads_record : 16u;
   // The passes of a tile follow each other in depth order; pixel state carries over.
   for (int p = 0; p < a.num_passes; ++p) {
   tile_segment(a, p, t);
@@ -568,7 +569,7 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
       }
       if (!hit) continue;  // wave-uniform: set only for bands with a hit
       const float val = reduce9(g, lane);
-      if (slot >= 0 && val != 0.0f) unsafeAtomicAdd(grads + size_t(gi_lds[j]) * 16 + slot, val);
+      if (slot >= 0 && val != 0.0f) unsafeAtomicAdd(grads + size_t(gi_lds[j]) * record + slot, val);
     }
   }
   }
