@@ -152,6 +152,17 @@ def copy_calibration(mib: int):
     ctx.close()
 
 
+def tune(args, batch, gsb, shb, tgt, grads, shape, shg):
+    """Sets the batch's knobs from --tune, or times candidates (autotune)."""
+    if args.tune:
+        batch.set_tuning([int(x) for x in args.tune.split(",")])
+    elif not args.no_autotune:
+        batch.autotune(gsb, shb, tgt, 0.0, grads, shape, shg)
+    tuning = batch.tuning()
+    log("tuning: " + ",".join(str(x) for x in tuning))
+    return tuning
+
+
 def train_loop(args):
     """The training workload: 3M Gaussians (64 clustered blobs), SH3, 64 views at 1600x1066; each
     step is the batched forward + backward of all views (L1) followed by the fused
@@ -179,9 +190,7 @@ def train_loop(args):
         ctx.adam_sh(float(t[0]), n, lr, gsb, shb, grads, shape, shg, moments)
 
     step()
-    if not args.no_autotune:
-        tile_batch, chunk = batch.autotune(gsb, shb, tgt, 0.0, grads, shape, shg)
-        log(f"autotune: tile_batch={tile_batch} chunk={chunk}")
+    tune(args, batch, gsb, shb, tgt, grads, shape, shg)
     for _ in range(args.warmup):
         step()
     ctx.sync()
@@ -239,6 +248,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tune", default="", help="tile_batch,chunk,split_first,split_later,split_backward to set "
+                    "instead of autotuning (the 'tuning' line of an earlier run)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0, help="only time a device copy of this many MiB")
     ap.add_argument("--train", action="store_true", help="the 3M-Gaussian, 64-view training loop instead")
@@ -268,10 +279,7 @@ def main():
         batch.set_passes(args.passes, args.growth)
     step = lambda: batch.step(gsb, shb, tgt, 0.0, grads, shape, shg, None, True)  # noqa: E731
     step()
-    tile_batch, chunk = 1, 0
-    if not args.no_autotune:
-        tile_batch, chunk = batch.autotune(gsb, shb, tgt, 0.0, grads, shape, shg)
-        log(f"autotune: tile_batch={tile_batch} chunk={chunk}")
+    tuning = tune(args, batch, gsb, shb, tgt, grads, shape, shg)
     for _ in range(args.warmup):
         step()
     ctx.sync()
@@ -337,7 +345,7 @@ def main():
             "stage_ms": {k: round(v, 4) for k, v in timed.items()},
             "stage_GBps": {k: round(v, 1) for k, v in stage_bw.items()},
             "backward_hbm_frac": round(bwd_achieved / HBM_PEAK_GBS, 4),
-            "graph": True, "tile_batch": tile_batch, "chunk": chunk,
+            "graph": True, "tuning": list(tuning),
             "depth_passes": args.passes or "default",
         },
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -285,8 +285,7 @@ int gsr_bin_sort(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, uint3
   launch_zero(ctx->stream, overflow, sizeof(uint32_t));
   launch_prep(ctx->stream, dview, 1, splats, n, keys, rects);
   SortScratch sc{keys_alt, vals, vals_alt, hist, piece};
-  launch_depth_sort(ctx->stream, keys, n, 1, sc);
-  launch_bin_gather(ctx->stream, 1, n, vals, rects, rect_sorted);
+  launch_depth_sort(ctx->stream, keys, n, 1, sc, rects, rect_sorted);
   BinScratch bs{matrix, tile_tot, rect_sorted, run};
   BinPass pass{0, n, nullptr, ranges, ebase, entries, true};
   launch_bin_count(ctx->stream, dview, &host, 1, n, chunk, bs, pass);

@@ -236,8 +236,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   launch_project(s, b->d_cams, b->d_views, b->V, gs, sh, b->deg, b->n, b->splats, b->keys, b->rects, nullptr);
   mark();
   SortScratch sc{b->keys_alt, b->vals, b->vals_alt, b->hist, b->piece};
-  launch_depth_sort(s, b->keys, b->n, b->V, sc);
-  launch_bin_gather(s, b->V, b->n, b->vals, b->rects, b->rect_sorted);
+  launch_depth_sort(s, b->keys, b->n, b->V, sc, b->rects, b->rect_sorted);
   mark();
   RasterArgs a{};
   a.views = b->d_views;
@@ -626,6 +625,16 @@ int gsr_batch_set_split(gsr_batch* b, uint32_t forward_first, uint32_t forward_l
     b->split_backward = backward;
     drop_graph(b);  // the thresholds are launch arguments of the captured step
   }
+  return GSR_OK;
+}
+
+int gsr_batch_tuning(gsr_batch* b, uint32_t* tuning) {
+  if (!b || !tuning) return GSR_E_INVALID;
+  tuning[0] = b->tile_batch;
+  tuning[1] = b->chunk;
+  tuning[2] = b->split_first;
+  tuning[3] = b->split_later;
+  tuning[4] = b->split_backward;
   return GSR_OK;
 }
 

@@ -70,9 +70,11 @@ struct SortScratch {
 };
 size_t radix_hist_elems(uint32_t n, uint32_t num_views);
 size_t scan_piece_elems(size_t seg_len, uint32_t num_segs);
-// Stable sort of (keys[v][i], i) per view by key; result indices in vals (and
-// keys sorted in place).  Four 8-bit passes.
-void launch_depth_sort(hipStream_t s, uint32_t* keys, uint32_t n, uint32_t num_views, const SortScratch& sc);
+// Stable sort of (keys[v][i], i) per view by key; result indices in vals, and
+// rect_sorted[v][p] = rects[v][vals[v][p]] (the sorted keys themselves are not
+// kept: keys is scratch afterwards).  Four 8-bit passes.
+void launch_depth_sort(hipStream_t s, uint32_t* keys, uint32_t n, uint32_t num_views, const SortScratch& sc,
+                       const uint2* rects, uint2* rect_sorted);
 
 struct BinScratch {
   uint32_t* matrix;      // counts of one pass: [view][chunk][tile] at chunks * tile_base
@@ -90,9 +92,6 @@ struct BinPass {
   uint64_t* entries;    // [V] entries per view in this pass
   bool first;           // first pass of the step: order slots start at 0
 };
-// rects in depth order (once per step, before the passes).
-void launch_bin_gather(hipStream_t s, uint32_t num_views, uint32_t n, const uint32_t* vals, const uint2* rects,
-                       uint2* rect_sorted);
 // Pass part 1: per-(chunk, tile) counts, tile totals, ranges, entry counts and
 // entry bases.  host_views: the views on the host (grid sizing).
 void launch_bin_count(hipStream_t s, const ViewDesc* views, const ViewDesc* host_views, uint32_t num_views,

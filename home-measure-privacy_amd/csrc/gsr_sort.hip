@@ -50,11 +50,18 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
   __syncthreads();
   const size_t base = size_t(v) * n + size_t(c) * kSortChunk;
   const uint32_t len = min(uint32_t(kSortChunk), n - c * kSortChunk);
-  for (uint32_t j0 = 0; j0 < uint32_t(kSortChunk); j0 += kSortThreads) {
-    uint32_t j = j0 + threadIdx.x;
-    bool active = j < len;
-    uint32_t d = active ? (keys[base + j] >> shift) & (kBins - 1) : 0u;
-    uint64_t peers = match_digit(d, active);
+  // All of the thread's loads are issued before the first is used.
+  uint32_t kr[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const uint32_t j = uint32_t(r) * kSortThreads + threadIdx.x;
+    kr[r] = j < len ? keys[base + j] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const bool active = uint32_t(r) * kSortThreads + threadIdx.x < len;
+    const uint32_t d = (kr[r] >> shift) & (kBins - 1);
+    const uint64_t peers = match_digit(d, active);
     if (active && (peers & lanemask_lt()) == 0) atomicAdd(&h[d], uint32_t(__popcll(peers)));
   }
   __syncthreads();
@@ -65,12 +72,16 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
 // chunk and ranks them in 16 rounds of 64 with ballot peer masks; the chunk is
 // then reordered by digit in LDS (stable: wave, round, lane order is index
 // order) and written out digit run by digit run, so consecutive threads store
-// to consecutive addresses.
+// to consecutive addresses.  In the last pass (rect_sorted set) the sorted keys
+// are not needed: each slot gets its Gaussian's tile rectangle instead,
+// rect_sorted[v][p] = rects[v][vals_out[v][p]] (empty rectangles mark rejected
+// Gaussians), so the binning blocks read rectangles coalesced in depth order.
 __device__ inline uint32_t block_exclusive_scan256(uint32_t x, uint32_t* tmp);
 
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t n, uint32_t chunks,
-    int shift, const uint32_t* __restrict__ hist, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out) {
+    int shift, const uint32_t* __restrict__ hist, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+    const uint2* __restrict__ rects, uint2* __restrict__ rect_sorted) {
   __shared__ uint32_t wcnt[4][kBins];  // per-wave digit counts, then the waves' local digit bases
   __shared__ uint32_t lstart[kBins];   // first local slot of each digit
   __shared__ uint32_t gpos[kBins];     // first output slot of each digit for this chunk
@@ -86,19 +97,24 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   const uint32_t first = c * kSortChunk + w * kWaveItems;
   uint32_t kr[kSortItems], vr[kSortItems], dl[kSortItems];
   const uint64_t lt = lanemask_lt();
+  // All loads first, so a wave has its 16 rounds in flight at once.
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    uint32_t j = first + r * 64 + lane;
-    bool active = j < n;
-    uint32_t key = active ? keys_in[seg + j] : 0u;
-    uint32_t val = active ? (vals_in ? vals_in[seg + j] : j) : 0u;
+    const uint32_t j = first + r * 64 + lane;
+    const bool active = j < n;
+    kr[r] = active ? keys_in[seg + j] : 0u;
+    vr[r] = active ? (vals_in ? vals_in[seg + j] : j) : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const uint32_t j = first + r * 64 + lane;
+    const bool active = j < n;
+    const uint32_t key = kr[r];
     uint32_t d = (key >> shift) & (kBins - 1);
     uint64_t peers = match_digit(d, active);
     uint32_t rank = __popcll(peers & lt);
     uint32_t before = active ? wcnt[w][d] : 0u;
     if (active && rank == 0) wcnt[w][d] = before + uint32_t(__popcll(peers));
-    kr[r] = key;
-    vr[r] = val;
     dl[r] = active ? (d << 16) | (before + rank) : 0xFFFFFFFFu;
   }
   __syncthreads();
@@ -130,8 +146,12 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t key = lkeys[li];
     const uint32_t d = (key >> shift) & (kBins - 1);
     const uint32_t dst = gpos[d] + (li - lstart[d]);
-    keys_out[seg + dst] = key;
-    vals_out[seg + dst] = lvals[li];
+    const uint32_t val = lvals[li];
+    vals_out[seg + dst] = val;
+    if (rect_sorted)
+      rect_sorted[seg + dst] = rects[seg + val];
+    else
+      keys_out[seg + dst] = key;
   }
 }
 
@@ -247,16 +267,6 @@ struct Rect {
 
 __device__ inline Rect unpack_rect(uint2 rc) {
   return Rect{rc.x & 0xFFFFu, rc.x >> 16, rc.y & 0xFFFFu, rc.y >> 16};
-}
-
-// rects in depth order: sorted[v][p] = rects[v][vals[v][p]] (empty rectangles
-// mark rejected Gaussians), so band blocks read them coalesced.
-__global__ __launch_bounds__(256) void bin_gather_kernel(const uint32_t* __restrict__ vals,
-                                                         const uint2* __restrict__ rects, uint32_t n,
-                                                         uint2* __restrict__ sorted) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t seg = size_t(blockIdx.y) * n;
-  if (p < n) sorted[seg + p] = rects[seg + vals[seg + p]];
 }
 
 struct Band {
@@ -594,7 +604,8 @@ size_t scan_piece_elems(size_t seg_len, uint32_t num_segs) {
   return size_t(num_segs) * ((seg_len + kScanPiece - 1) / kScanPiece);
 }
 
-void launch_depth_sort(hipStream_t s, uint32_t* keys, uint32_t n, uint32_t num_views, const SortScratch& sc) {
+void launch_depth_sort(hipStream_t s, uint32_t* keys, uint32_t n, uint32_t num_views, const SortScratch& sc,
+                       const uint2* rects, uint2* rect_sorted) {
   if (n == 0 || num_views == 0) return;
   const uint32_t chunks = (n + kSortChunk - 1) / kSortChunk;
   dim3 grid(chunks, num_views);
@@ -602,14 +613,15 @@ void launch_depth_sort(hipStream_t s, uint32_t* keys, uint32_t n, uint32_t num_v
   uint32_t* kout = sc.keys_alt;
   const uint32_t* vin = nullptr;
   uint32_t* vout = sc.vals_alt;
-  for (int pass = 0; pass < 32 / kRadixBits; ++pass) {
+  constexpr int kPasses = 32 / kRadixBits;
+  for (int pass = 0; pass < kPasses; ++pass) {
     int shift = pass * kRadixBits;
+    const bool last = pass == kPasses - 1;
     hipLaunchKernelGGL(radix_hist_kernel, grid, dim3(kSortThreads), 0, s, kin, n, chunks, shift, sc.hist);
     scan_segments(s, sc.hist, size_t(kBins) * chunks, num_views, sc.piece);
     hipLaunchKernelGGL(radix_scatter_kernel, grid, dim3(kSortThreads), 0, s, kin, vin, n, chunks, shift, sc.hist,
-                       kout, vout);
-    // Ping-pong: after an even number of passes the keys are back in `keys` and
-    // the values in `vals`.
+                       kout, vout, rects, last ? rect_sorted : nullptr);
+    // Ping-pong: after an even number of passes the values are in `vals`.
     uint32_t* kt = kin; kin = kout; kout = kt;
     vin = vout;
     vout = (vout == sc.vals_alt) ? sc.vals : sc.vals_alt;
@@ -647,13 +659,6 @@ uint32_t slice_chunks(const BinPass& p, uint32_t chunk) {
   return p.pend > p.pbeg ? (p.pend - p.pbeg + chunk - 1) / chunk : 0;
 }
 }  // namespace
-
-void launch_bin_gather(hipStream_t s, uint32_t num_views, uint32_t n, const uint32_t* vals, const uint2* rects,
-                       uint2* rect_sorted) {
-  if (n == 0 || num_views == 0) return;
-  hipLaunchKernelGGL(bin_gather_kernel, dim3((n + 255) / 256, num_views), dim3(256), 0, s, vals, rects, n,
-                     rect_sorted);
-}
 
 void launch_bin_count(hipStream_t s, const ViewDesc* views, const ViewDesc* host_views, uint32_t num_views,
                       uint32_t n, uint32_t chunk, const BinScratch& bs, const BinPass& pass) {

@@ -25,6 +25,7 @@ EXPORTS = (
     "gsr_shape", "gsr_adam", "gsr_adam_sh", "gsr_stats", "gsr_batch_create", "gsr_batch_destroy", "gsr_batch_set_cameras",
     "gsr_batch_step", "gsr_batch_entries", "gsr_batch_configure", "gsr_batch_autotune", "gsr_batch_profile",
     "gsr_batch_set_passes", "gsr_loss_grad_window", "gsr_batch_set_ssim_window", "gsr_batch_set_split",
+    "gsr_batch_tuning",
 )
 SSIM_BOX7, SSIM_GAUSS11 = 7, 11
 STAGES = ("project", "depth_sort", "bin_count", "bin_emit", "forward", "loss_grad", "backward", "chain")
@@ -93,6 +94,7 @@ def load():
         "gsr_batch_set_passes": (i32, [P, u32, u32]),
         "gsr_loss_grad_window": (i32, [P, ctypes.POINTER(Camera), P, P, f32, i32, P]),
         "gsr_batch_set_ssim_window": (i32, [P, i32]),
+        "gsr_batch_tuning": (i32, [P, ctypes.POINTER(u32)]),
         "gsr_batch_set_split": (i32, [P, u32, u32, u32]),
     }
     for name, (res, args) in sig.items():
@@ -287,6 +289,17 @@ class Batch:
         """Split thresholds (0 = never): the forward's first slice (by the previous
         step's walk), its later slices (by list length) and the backward."""
         self.ctx._check(self.lib.gsr_batch_set_split(self.h, forward_first, forward_later, backward))
+
+    def tuning(self) -> tuple:
+        """(tile_batch, chunk, split_first, split_later, split_backward) in effect."""
+        out = (ctypes.c_uint32 * 5)()
+        self.ctx._check(self.lib.gsr_batch_tuning(self.h, out))
+        return tuple(int(x) for x in out)
+
+    def set_tuning(self, tuning):
+        tb, ch, sf, sl, sb = tuning
+        self.configure(tb, ch)
+        self.set_split(sf, sl, sb)
 
     def set_ssim_window(self, window: int):
         self.ctx._check(self.lib.gsr_batch_set_ssim_window(self.h, window))

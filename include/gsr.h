@@ -199,6 +199,10 @@ int gsr_batch_configure(gsr_batch* batch, uint32_t tile_batch, uint32_t chunk);
  * walked).  0 = never; defaults 0, 0 and 1024.  gsr_batch_autotune picks all
  * three.  Results do not depend on them. */
 int gsr_batch_set_split(gsr_batch* batch, uint32_t forward_first, uint32_t forward_later, uint32_t backward);
+/* The knobs in effect (after gsr_batch_autotune, say), so a later run can set
+ * them directly: tuning[5] = tile_batch, chunk, and the three split thresholds
+ * in gsr_batch_set_split's order. */
+int gsr_batch_tuning(gsr_batch* batch, uint32_t* tuning);
 /* SSIM window of the batched step's loss (default GSR_SSIM_BOX7). */
 int gsr_batch_set_ssim_window(gsr_batch* batch, int window);
 /* Depth-slice passes of the batched step (1 to 8; default 2 with growth 8 for
