@@ -43,10 +43,11 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
   float gmean[3] = {0.f, 0.f, 0.f};
   float gcol[3] = {0.f, 0.f, 0.f};
   float gop = 0.0f;
-  for (uint32_t v = 0; v < num_views; ++v) {
+  // Most (Gaussian, view) records are zero, so the walk over views is bound by
+  // load latency: the records of kAhead views are loaded before any is used.
+  constexpr uint32_t kAhead = 4;
+  auto view_step = [&](uint32_t v, const float (&gr)[9]) __attribute__((always_inline)) {
     float4* r = reinterpret_cast<float4*>(grads + size_t(v) * stride + size_t(i) * record);
-    const float4 ga4 = r[0], gb4 = r[1], gc4 = r[2];
-    const float gr[9] = {ga4.x, ga4.y, ga4.z, ga4.w, gb4.x, gb4.y, gb4.z, gb4.w, gc4.x};
     gcol[0] += gr[0]; gcol[1] += gr[1]; gcol[2] += gr[2];
     gop += gr[3];
     bool screen = gr[4] != 0.0f || gr[5] != 0.0f || gr[6] != 0.0f || gr[7] != 0.0f || gr[8] != 0.0f;
@@ -61,13 +62,13 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
       if (m && __lane_id() == uint32_t(__ffsll(static_cast<unsigned long long>(m)) - 1))
         atomicAdd(touched + v, uint32_t(__popcll(m)));
     }
-    if (!screen && !colour) continue;
+    if (!screen && !colour) return;
     const CamF cam = load_cam(cams[v]);
     float p[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
       p[k] = cam.V[k][0] * mean[0] + cam.V[k][1] * mean[1] + cam.V[k][2] * mean[2] + cam.t[k];
-    if (p[2] < 0.2f) continue;
+    if (p[2] < 0.2f) return;
     // Camera-frame covariance, as project_cov.
     float T[3][3], C[3][3];
 #pragma unroll
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
     const float cb = j0[0] * Cj1[0] + j0[1] * Cj1[1] + j0[2] * Cj1[2];
     const float cc = j1[0] * Cj1[0] + j1[1] * Cj1[1] + j1[2] * Cj1[2] + 0.3f;
     const float D = ca * cc - cb * cb;
-    if (!(D > 0.0f)) continue;
+    if (!(D > 0.0f)) return;
     if (screen) {
       const float gu = gr[4], gv = gr[5], gA = gr[6], gB = gr[7], gC = gr[8];
       // Conic (A, B, C) = (c, -b, a) / D.
@@ -163,6 +164,22 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
         for (int c = 0; c < 3; ++c) gsh[c] += gr[c];
       }
     }
+  };
+  for (uint32_t v0 = 0; v0 < num_views; v0 += kAhead) {
+    float rec[kAhead][9];
+#pragma unroll
+    for (uint32_t k = 0; k < kAhead; ++k) {
+      // Past the last view the last view's record is loaded again and not used.
+      const uint32_t v = min(v0 + k, num_views - 1);
+      const float4* r = reinterpret_cast<const float4*>(grads + size_t(v) * stride + size_t(i) * record);
+      const float4 a = r[0], b = r[1];
+      rec[k][0] = a.x; rec[k][1] = a.y; rec[k][2] = a.z; rec[k][3] = a.w;
+      rec[k][4] = b.x; rec[k][5] = b.y; rec[k][6] = b.z; rec[k][7] = b.w;
+      rec[k][8] = reinterpret_cast<const float*>(r)[8];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kAhead; ++k)
+      if (v0 + k < num_views) view_step(v0 + k, rec[k]);
   }
   // Sigma = M M^T, M = R diag(s): dL/dM = 2 Gw M; log-scale and rotation.
   const float s[3] = {expf(q[3]), expf(q[4]), expf(q[5])};
