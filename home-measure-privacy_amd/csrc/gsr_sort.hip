@@ -443,6 +443,17 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t x) {
 
 constexpr uint32_t kEmitQueue = 128;
 constexpr uint32_t kEmitFixedLds = kEmitQueue * 16 + 64 * 4;  // queue records + owner marks
+// Workgroup L runs on die L mod 8 (eight dies, each with its own L2).  A tile's
+// slots from neighbouring chunks are neighbours in the order, so each die takes
+// kEmitDieRun consecutive chunks of a band: their short runs then meet in one L2
+// and leave it as whole lines.  Groups of 8 * kEmitDieRun workgroups interleave
+// the dies, which keeps their loads even; the tail maps to itself.
+constexpr uint32_t kEmitDieRun = 8;
+__device__ inline uint32_t emit_block(uint32_t L, uint32_t total) {
+  constexpr uint32_t group = 8 * kEmitDieRun;
+  if (L >= total / group * group) return L;
+  return L / group * group + (L % 8) * kEmitDieRun + (L / 8) % kEmitDieRun;
+}
 
 __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict__ views, uint32_t n, uint32_t chunk,
                                                       uint32_t pbeg, uint32_t pend, uint32_t chunks,
@@ -454,9 +465,11 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
                                                       uint32_t* __restrict__ order, uint64_t capacity,
                                                       uint32_t* __restrict__ overflow) {
   extern __shared__ uint4 lds_quads[];
-  const uint32_t v = blockIdx.z, c = blockIdx.x, lane = threadIdx.x;
+  const uint32_t X = gridDim.x, XY = gridDim.x * gridDim.y;
+  const uint32_t logical = emit_block(blockIdx.x + X * blockIdx.y + XY * blockIdx.z, XY * gridDim.z);
+  const uint32_t v = logical / XY, c = logical % X, lane = threadIdx.x;
   const ViewDesc vd = views[v];
-  const Band bd = band_of(vd, blockIdx.y, kEmitBandTiles);
+  const Band bd = band_of(vd, (logical % XY) / X, kEmitBandTiles);
   if (!bd.valid) return;
   const uint32_t tx = bd.tx, ntile = (bd.r1 - bd.r0) * tx;
   // queue record: band-local tile of the rectangle's corner, width, pair count
