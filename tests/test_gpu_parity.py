@@ -109,7 +109,10 @@ def test_bin_sort_bit_exact_300k_1280x720(ctx):
     assert order.size > 10 * sc.n
 
 
-@pytest.mark.parametrize("w,h,n", [(1280, 720, 60000), (250, 130, 5000), (17, 33, 300), (1, 1, 50)])
+# 16384 px: 1024 tiles across, the widest view whose rectangles travel packed
+# through the depth sort; 16400 px (1025 tiles) takes the gathering path.
+@pytest.mark.parametrize("w,h,n", [(1280, 720, 60000), (250, 130, 5000), (17, 33, 300), (1, 1, 50),
+                                   (16384, 40, 3000), (16400, 40, 3000)])
 def test_bin_sort_bit_exact_sizes(ctx, w, h, n):
     sc = scenes.config(1, n=n, with_targets=False)
     cam = scenes.look_at(3.0 * np.array([0.3, -0.4, 0.866]), [0, 0, 0], w, h, 60.0)
