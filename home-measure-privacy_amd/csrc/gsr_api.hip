@@ -27,6 +27,10 @@ struct gsr_ctx {
   uint64_t* pinned = nullptr;
 };
 
+namespace gsr {
+hipError_t ctx_malloc(gsr_ctx* ctx, void** p, size_t bytes);
+}  // namespace gsr
+
 namespace {
 
 thread_local std::string g_create_error;
@@ -68,7 +72,8 @@ int pooled(gsr_ctx* ctx, const char* name, size_t bytes, void** out) {
   }
   size_t sz = std::max<size_t>(256, ((bytes + bytes / 4) + 255) / 256 * 256);
   void* p = nullptr;
-  if (hipMalloc(&p, sz) != hipSuccess) return fail(ctx, GSR_E_NOMEM, "out of device memory (%zu bytes for %s)", sz, name);
+  if (gsr::ctx_malloc(ctx, &p, sz) != hipSuccess)
+    return fail(ctx, GSR_E_NOMEM, "out of device memory (%zu bytes for %s)", sz, name);
   ctx->pool[name] = {p, sz};
   *out = p;
   return GSR_OK;
@@ -156,7 +161,7 @@ int gsr_create(int device, gsr_ctx** out) {
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
   if (ctx->num_cus <= 0) ctx->num_cus = 256;
   if ((e = hipHostMalloc(reinterpret_cast<void**>(&ctx->pinned), 4096, hipHostMallocDefault)) != hipSuccess) {
-    hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return fail(nullptr, GSR_E_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
   }
@@ -166,11 +171,12 @@ int gsr_create(int device, gsr_ctx** out) {
 
 void gsr_destroy(gsr_ctx* ctx) {
   if (!ctx) return;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
-  for (auto& kv : ctx->pool) hipFree(kv.second.first);
-  if (ctx->pinned) hipHostFree(ctx->pinned);
-  hipStreamDestroy(ctx->stream);
+  // Teardown: nothing to report to a caller that is destroying the context.
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->pool) (void)hipFree(kv.second.first);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
 
@@ -180,7 +186,7 @@ int gsr_alloc(gsr_ctx* ctx, size_t bytes, void** dptr) {
   if (!ctx || !dptr) return fail(ctx, GSR_E_INVALID, "NULL argument");
   const size_t sz = (std::max<size_t>(16, bytes) + 15) / 16 * 16;
   void* p = nullptr;
-  if (hipMalloc(&p, sz) != hipSuccess) return fail(ctx, GSR_E_NOMEM, "out of device memory (%zu bytes)", sz);
+  if (gsr::ctx_malloc(ctx, &p, sz) != hipSuccess) return fail(ctx, GSR_E_NOMEM, "out of device memory (%zu bytes)", sz);
   launch_zero(ctx->stream, p, sz);
   GSR_LAUNCHED(ctx);
   GSR_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -438,4 +444,10 @@ namespace gsr {
 hipStream_t ctx_stream(gsr_ctx* ctx) { return ctx->stream; }
 int ctx_num_cus(gsr_ctx* ctx) { return ctx->num_cus; }
 int ctx_fail(gsr_ctx* ctx, int code, const char* msg) { return fail(ctx, code, "%s", msg); }
+// Device memory on the context's device, whichever device the calling thread
+// has current (the current device is per thread).
+hipError_t ctx_malloc(gsr_ctx* ctx, void** p, size_t bytes) {
+  const hipError_t e = hipSetDevice(ctx->device);
+  return e != hipSuccess ? e : hipMalloc(p, bytes);
+}
 }  // namespace gsr

@@ -27,6 +27,7 @@ namespace gsr {
 hipStream_t ctx_stream(gsr_ctx* ctx);
 int ctx_num_cus(gsr_ctx* ctx);
 int ctx_fail(gsr_ctx* ctx, int code, const char* msg);
+hipError_t ctx_malloc(gsr_ctx* ctx, void** p, size_t bytes);
 }  // namespace gsr
 
 constexpr uint32_t kQueueBase = 32;
@@ -98,7 +99,7 @@ struct gsr_batch {
 namespace {
 
 void dfree(void* p) {
-  if (p) hipFree(p);
+  if (p) (void)hipFree(p);
 }
 
 template <class T>
@@ -106,7 +107,7 @@ int dalloc(gsr_batch* b, T** p, size_t count) {
   dfree(*p);
   *p = nullptr;
   size_t bytes = std::max<size_t>(256, count * sizeof(T));
-  if (hipMalloc(reinterpret_cast<void**>(p), bytes) != hipSuccess) {
+  if (ctx_malloc(b->ctx, reinterpret_cast<void**>(p), bytes) != hipSuccess) {
     *p = nullptr;
     return ctx_fail(b->ctx, GSR_E_NOMEM, "out of device memory for a batch buffer");
   }
@@ -115,8 +116,8 @@ int dalloc(gsr_batch* b, T** p, size_t count) {
 }
 
 void drop_graph(gsr_batch* b) {
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
   b->graph_key.clear();
@@ -224,7 +225,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   float* image = images ? images : b->image;
   int stage = 0;
   auto mark = [&]() {
-    if (prof) hipEventRecord(prof->ev[stage++], s);
+    if (prof) (void)hipEventRecord(prof->ev[stage++], s);  // a failed record fails the timing query
   };
   launch_zero(s, b->flags, kFlagWords * sizeof(uint32_t));
   // The first slice's walk of the previous step estimates this step's: the
@@ -267,21 +268,22 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   float t_count = 0.f, t_emit = 0.f, t_fwd = 0.f;
   hipEvent_t e0, e1, e2, e3;
   if (prof) {
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventCreate(&e2);
-    hipEventCreate(&e3);
+    // Profiling only: a failed event call leaves the time at 0 and the step as it is.
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventCreate(&e2);
+    (void)hipEventCreate(&e3);
   }
   for (uint32_t p = 0; p < b->num_passes; ++p) {
     BinPass pass{p ? b->pass_end[p - 1] : 0u, b->pass_end[p], p ? b->done : nullptr,
                  b->ranges + size_t(p) * b->total_tiles * 2, b->ebase + size_t(p) * b->V,
                  b->entries + size_t(p) * b->V, p == 0};
-    if (prof) hipEventRecord(e0, s);
+    if (prof) (void)hipEventRecord(e0, s);
     launch_bin_count(s, b->d_views, b->views.data(), b->V, b->n, b->chunk, bs, pass);
-    if (prof) hipEventRecord(e1, s);
+    if (prof) (void)hipEventRecord(e1, s);
     launch_bin_emit(s, b->d_views, b->views.data(), b->V, b->n, b->chunk, b->vals, bs, pass, b->order, b->order_cap,
                     b->flags);
-    if (prof) hipEventRecord(e2, s);
+    if (prof) (void)hipEventRecord(e2, s);
     a.pass = int(p);
     a.queue = b->flags + kQueueBase + 9 * p;
     const uint32_t split_cost = p ? b->split_later : b->split_first;
@@ -306,14 +308,14 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
     launch_forward(s, a, 1, b->grid_fwd);
     if (a.done && p + 1 < b->num_passes) launch_done_update(s, b->total_tiles, b->band_open, b->done, p == 0);
     if (prof) {
-      hipEventRecord(e3, s);
-      hipEventSynchronize(e3);
+      (void)hipEventRecord(e3, s);
+      (void)hipEventSynchronize(e3);
       float x;
-      hipEventElapsedTime(&x, e0, e1);
+      if (hipEventElapsedTime(&x, e0, e1) != hipSuccess) x = 0.f;
       t_count += x;
-      hipEventElapsedTime(&x, e1, e2);
+      if (hipEventElapsedTime(&x, e1, e2) != hipSuccess) x = 0.f;
       t_emit += x;
-      hipEventElapsedTime(&x, e2, e3);
+      if (hipEventElapsedTime(&x, e2, e3) != hipSuccess) x = 0.f;
       t_fwd += x;
     }
   }
@@ -321,7 +323,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
     prof->binned_ms[0] = t_count;
     prof->binned_ms[1] = t_emit;
     prof->binned_ms[2] = t_fwd;
-    for (hipEvent_t e : {e0, e1, e2, e3}) hipEventDestroy(e);
+    for (hipEvent_t e : {e0, e1, e2, e3}) (void)hipEventDestroy(e);
   }
   a.walk = nullptr;
   a.last = nullptr;
@@ -457,14 +459,14 @@ int gsr_batch_create(gsr_ctx* ctx, uint32_t num_views, const gsr_camera* cams, u
 
 void gsr_batch_destroy(gsr_batch* b) {
   if (!b) return;
-  hipStreamSynchronize(ctx_stream(b->ctx));
+  (void)hipStreamSynchronize(ctx_stream(b->ctx));
   drop_graph(b);
   void* ptrs[] = {b->d_cams, b->d_views, b->d_work, b->splats, b->keys, b->keys_alt, b->vals, b->vals_alt,
                   b->rects, b->rect_sorted, b->hist, b->piece, b->matrix, b->tile_tot, b->entries, b->ebase,
                   b->run, b->ranges, b->tstate, b->done, b->order, b->flags, b->image, b->dl, b->coef, b->vgrads,
                   b->band_open, b->walk, b->heavy, b->split, b->prev_walk};
   for (void* p : ptrs) dfree(p);
-  if (b->pinned) hipHostFree(b->pinned);
+  if (b->pinned) (void)hipHostFree(b->pinned);
   delete b;
 }
 
@@ -543,31 +545,37 @@ int gsr_batch_profile(gsr_batch* b, const float* gs, const float* sh, const floa
   const size_t nt = std::max<size_t>(1, b->total_tiles);
   const size_t np = size_t(b->num_passes) * nt;
   std::vector<uint32_t> tl(np), tw(np), touched(b->V);
-  if (hipMalloc(reinterpret_cast<void**>(&prof.last), size_t(kMaxPasses) * nt * 4) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void**>(&prof.touched), b->V * 4) != hipSuccess) {
+  if (ctx_malloc(b->ctx, reinterpret_cast<void**>(&prof.last), size_t(kMaxPasses) * nt * 4) != hipSuccess ||
+      ctx_malloc(b->ctx, reinterpret_cast<void**>(&prof.touched), b->V * 4) != hipSuccess) {
     dfree(prof.last);
     return ctx_fail(b->ctx, GSR_E_NOMEM, "profile buffers");
   }
   launch_zero(s, prof.touched, b->V * 4);
-  for (auto& e : prof.ev) hipEventCreate(&e);
+  hipError_t e = hipSuccess;
+  auto keep = [&](hipError_t r) {  // the first failure wins
+    if (e == hipSuccess) e = r;
+  };
+  for (auto& ev : prof.ev) keep(hipEventCreate(&ev));
   // Stage times come from a step that runs the same kernels as a regular step;
   // the counters from a second one that also tracks each tile's last hit.
   uint32_t* last = prof.last;
   prof.last = nullptr;
-  issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, &prof);
-  hipError_t e = hipStreamSynchronize(s);
+  if (e == hipSuccess) {
+    issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, &prof);
+    keep(hipStreamSynchronize(s));
+  }
   float binned[3] = {prof.binned_ms[0], prof.binned_ms[1], prof.binned_ms[2]};
   float marks[6] = {0, 0, 0, 0, 0, 0};
   if (e == hipSuccess) {
-    hipEventElapsedTime(marks + 0, prof.ev[0], prof.ev[1]);
-    hipEventElapsedTime(marks + 1, prof.ev[1], prof.ev[2]);
-    hipEventElapsedTime(marks + 2, prof.ev[3], prof.ev[4]);
-    hipEventElapsedTime(marks + 3, prof.ev[4], prof.ev[5]);
-    hipEventElapsedTime(marks + 4, prof.ev[5], prof.ev[6]);
+    keep(hipEventElapsedTime(marks + 0, prof.ev[0], prof.ev[1]));
+    keep(hipEventElapsedTime(marks + 1, prof.ev[1], prof.ev[2]));
+    keep(hipEventElapsedTime(marks + 2, prof.ev[3], prof.ev[4]));
+    keep(hipEventElapsedTime(marks + 3, prof.ev[4], prof.ev[5]));
+    keep(hipEventElapsedTime(marks + 4, prof.ev[5], prof.ev[6]));
     prof.last = last;
     launch_zero(s, prof.touched, b->V * 4);
     issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, &prof);
-    e = hipStreamSynchronize(s);
+    keep(hipStreamSynchronize(s));
   }
   if (e == hipSuccess) {
     // Stage order: project, depth sort, bin count, bin emit, forward, loss, backward, chain.
@@ -579,12 +587,14 @@ int gsr_batch_profile(gsr_batch* b, const float* gs, const float* sh, const floa
     stage_ms[5] = marks[2];
     stage_ms[6] = marks[3];
     stage_ms[7] = marks[4];
+    std::vector<uint64_t> pe(size_t(b->num_passes) * b->V), ent(b->V, 0);
     if (counters) {
-      hipMemcpy(tl.data(), prof.last, np * 4, hipMemcpyDeviceToHost);
-      hipMemcpy(tw.data(), b->walk, np * 4, hipMemcpyDeviceToHost);
-      hipMemcpy(touched.data(), prof.touched, b->V * 4, hipMemcpyDeviceToHost);
-      std::vector<uint64_t> pe(size_t(b->num_passes) * b->V), ent(b->V, 0);
-      hipMemcpy(pe.data(), b->entries, pe.size() * 8, hipMemcpyDeviceToHost);
+      keep(hipMemcpy(tl.data(), prof.last, np * 4, hipMemcpyDeviceToHost));
+      keep(hipMemcpy(tw.data(), b->walk, np * 4, hipMemcpyDeviceToHost));
+      keep(hipMemcpy(touched.data(), prof.touched, b->V * 4, hipMemcpyDeviceToHost));
+      keep(hipMemcpy(pe.data(), b->entries, pe.size() * 8, hipMemcpyDeviceToHost));
+    }
+    if (counters && e == hipSuccess) {
       for (uint32_t p = 0; p < b->num_passes; ++p)
         for (uint32_t v = 0; v < b->V; ++v) ent[v] += pe[size_t(p) * b->V + v];
       for (uint32_t v = 0; v < b->V; ++v) {
@@ -610,8 +620,8 @@ int gsr_batch_profile(gsr_batch* b, const float* gs, const float* sh, const floa
       }
     }
   }
-  for (auto& ev : prof.ev) hipEventDestroy(ev);
-  dfree(prof.last);
+  for (auto& ev : prof.ev) (void)hipEventDestroy(ev);
+  dfree(last);  // prof.last is still null when the first step failed
   dfree(prof.touched);
   if (e != hipSuccess) return ctx_fail(b->ctx, GSR_E_HIP, "profiled step failed");
   return GSR_OK;
