@@ -182,6 +182,8 @@ def train_loop(args):
     # opacity / scale / rotation as trainScene (gpu.js:891)
     lr = {"color": 5e-3, "sh_rest": 2.5e-4, "mean": 1.6e-4, "opacity": 0.05, "scale": 0.005, "rotation": 0.001}
     batch = lib.Batch(ctx, sc.cams, n, 3)
+    if args.passes:
+        batch.set_passes(args.passes, args.growth)
     t = [0]
 
     def step():

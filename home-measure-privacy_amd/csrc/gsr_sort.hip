@@ -41,6 +41,11 @@ __device__ inline uint64_t match_digit(uint32_t d, bool active) {
 }
 
 // Digit histogram of one 4096-key chunk of one view: hist[v][digit][chunk].
+// Mantissa digits of the depth keys are nearly uniform, so one LDS atomic per
+// key rarely collides; the top digit (sign and high exponent bits) takes a
+// handful of values, so there lanes with equal digits are matched by ballots
+// first and add their count once (kMatch).
+template <bool kMatch>
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t* __restrict__ keys, uint32_t n,
                                                                    uint32_t chunks, int shift,
                                                                    uint32_t* __restrict__ hist) {
@@ -61,8 +66,12 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
   for (int r = 0; r < kSortItems; ++r) {
     const bool active = uint32_t(r) * kSortThreads + threadIdx.x < len;
     const uint32_t d = (kr[r] >> shift) & (kBins - 1);
-    const uint64_t peers = match_digit(d, active);
-    if (active && (peers & lanemask_lt()) == 0) atomicAdd(&h[d], uint32_t(__popcll(peers)));
+    if constexpr (kMatch) {
+      const uint64_t peers = match_digit(d, active);
+      if (active && (peers & lanemask_lt()) == 0) atomicAdd(&h[d], uint32_t(__popcll(peers)));
+    } else if (active) {
+      atomicAdd(&h[d], 1u);
+    }
   }
   __syncthreads();
   hist[(size_t(v) * kBins + threadIdx.x) * chunks + c] = h[threadIdx.x];
@@ -630,7 +639,10 @@ void launch_depth_sort(hipStream_t s, uint32_t* keys, uint32_t n, uint32_t num_v
   for (int pass = 0; pass < kPasses; ++pass) {
     int shift = pass * kRadixBits;
     const bool last = pass == kPasses - 1;
-    hipLaunchKernelGGL(radix_hist_kernel, grid, dim3(kSortThreads), 0, s, kin, n, chunks, shift, sc.hist);
+    if (shift == 32 - kRadixBits)
+      hipLaunchKernelGGL(radix_hist_kernel<true>, grid, dim3(kSortThreads), 0, s, kin, n, chunks, shift, sc.hist);
+    else
+      hipLaunchKernelGGL(radix_hist_kernel<false>, grid, dim3(kSortThreads), 0, s, kin, n, chunks, shift, sc.hist);
     scan_segments(s, sc.hist, size_t(kBins) * chunks, num_views, sc.piece);
     hipLaunchKernelGGL(radix_scatter_kernel, grid, dim3(kSortThreads), 0, s, kin, vin, n, chunks, shift, sc.hist,
                        kout, vout, rects, last ? rect_sorted : nullptr);
