@@ -652,7 +652,7 @@ int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const flo
                        float* grads, float* shape, float* sh_grad, uint32_t* tile_batch, uint32_t* chunk) {
   if (!b) return GSR_E_INVALID;
   const uint32_t tbs[] = {1, 2, 4, 8};
-  const uint32_t chunks[] = {1024, 2048, 4096};
+  const uint32_t chunks[] = {1024, 2048, 4096, 8192};
   double best = 1e30;
   uint32_t best_tb = b->tile_batch, best_ch = b->chunk;
   for (uint32_t ch : chunks) {
