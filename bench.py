@@ -40,7 +40,9 @@ def sub(buf: lib.Buffer, offset: int, nbytes: int) -> lib.Buffer:
 
 
 def stage_bytes(sc, counters, n):
-    """Algorithmic bytes per stage (compulsory traffic only), for the whole batch."""
+    """Algorithmic bytes per stage (compulsory traffic only), for the whole batch.
+    The step runs the L1 loss (lambda 0), whose gradient the backward forms from
+    the image and the target it reads anyway, so the loss stage moves nothing."""
     S = SH_BYTES if sc.sh_degree > 0 else 0
     V = len(sc.cams)
     P = np.array([c.pixels for c in sc.cams], dtype=np.float64)
@@ -50,7 +52,6 @@ def stage_bytes(sc, counters, n):
         "project": n * (64 + S) + V * n * 48,
         "bin_sort": float((n * 48 + E * 4 + T * 8).sum()),
         "forward": float((F * (4 + 48) + T * 8 + P * 12).sum()),
-        "loss_grad": float((P * 36).sum()),
         "backward": float((F * (4 + 48) + T * 8 + P * 12 + P * 12 + G * 36 * 2).sum()),
         "chain": n * (64 + S) + V * n * 36 + n * (40 + S),
     }
@@ -301,7 +302,7 @@ def main():
     stage_bw["bin_sort"] = bytes_["bin_sort"] / (
         (stage_ms["depth_sort"] + stage_ms["bin_count"] + stage_ms["bin_emit"]) * 1e-3) / 1e9
     timed = {k: v for k, v in stage_ms.items()}
-    dominant = max(("forward", "backward", "project", "chain", "loss_grad"), key=lambda k: timed[k])
+    dominant = max(("forward", "backward", "project", "chain"), key=lambda k: timed[k])
     sort_ms = timed["depth_sort"] + timed["bin_count"] + timed["bin_emit"]
     if sort_ms > timed[dominant]:
         dominant = "bin_sort"

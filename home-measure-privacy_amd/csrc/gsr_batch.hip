@@ -328,7 +328,10 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   a.walk = nullptr;
   a.last = nullptr;
   mark();
-  launch_loss_grad(s, b->d_views, b->views.data(), b->V, image, targets, lambda, b->coef, b->dl, b->ssim_window);
+  // Without the SSIM term the backward forms the L1 gradient itself.
+  if (lambda > 0.0f)
+    launch_loss_grad(s, b->d_views, b->views.data(), b->V, image, targets, lambda, b->coef, b->dl, b->ssim_window);
+  a.l1_target = lambda > 0.0f ? nullptr : targets;
   mark();
   a.split = b->split_backward ? b->split : nullptr;
   a.heavy_count = b->flags + 16 + kMaxPasses;

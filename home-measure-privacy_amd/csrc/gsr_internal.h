@@ -120,6 +120,9 @@ struct RasterArgs {
   float* image;          // concatenated views
   float* state;          // optional (T, last), single pass only
   const float* dl;       // backward: dLoss/dPixel, concatenated views
+  // backward, optional: the targets (concatenated views); dLoss/dPixel is then
+  // the L1 gradient sign(image - target) / (3 pixels) and dl is not read.
+  const float* l1_target;
   float* grads;          // backward: [V][n][16]
   uint64_t grads_view_stride;  // elements between views' gradient records (0: shared)
   uint32_t grads_record;       // floats per gradient record (0: the reference's 16)

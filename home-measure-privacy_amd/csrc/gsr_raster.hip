@@ -464,6 +464,8 @@ __device__ inline int reduce9_slot(uint32_t lane) {
   return lane == 2 ? 8 : -1;
 }
 
+__device__ inline float sign_f(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
+
 __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[4], uint32_t* gi_lds) {
   const uint32_t lane = __lane_id();
   Tile t;
@@ -487,9 +489,18 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
       fr[i] = a.image[pix * 3];
       fg[i] = a.image[pix * 3 + 1];
       fb[i] = a.image[pix * 3 + 2];
-      dr[i] = a.dl[pix * 3];
-      dg[i] = a.dl[pix * 3 + 1];
-      db[i] = a.dl[pix * 3 + 2];
+      if (a.l1_target) {
+        // The L1 gradient (lossGradient at lambda 0, gpu.js:389-392), as
+        // loss_grad_kernel computes it, without the round trip through dl.
+        const float n = float(t.width * t.height * 3u);
+        dr[i] = sign_f(fr[i] - a.l1_target[pix * 3]) / n;
+        dg[i] = sign_f(fg[i] - a.l1_target[pix * 3 + 1]) / n;
+        db[i] = sign_f(fb[i] - a.l1_target[pix * 3 + 2]) / n;
+      } else {
+        dr[i] = a.dl[pix * 3];
+        dg[i] = a.dl[pix * 3 + 1];
+        db[i] = a.dl[pix * 3 + 2];
+      }
     }
   }
   const float band0 = float(t.y0) + 0.5f;
@@ -659,7 +670,6 @@ __global__ __launch_bounds__(256) void ssim_coef_kernel(const ViewDesc* __restri
   }
 }
 
-__device__ inline float sign_f(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
 
 // LOSS_GRAD (gpu.js:383-409): dLoss/dpixel of (1 - lambda) L1 + lambda (1 - SSIM)
 // = (1 - lambda) sign(x - y) / (3P) - lambda / windows * sum over the windows

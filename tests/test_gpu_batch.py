@@ -30,7 +30,8 @@ def small_batch_scene(n=20000, views=5, seed=0):
     return sc
 
 
-def run_batch(ctx, sc, graph=True, tile_batch=0, chunk=0, steps=1, images=False, passes=None, growth=2, split=None):
+def run_batch(ctx, sc, graph=True, tile_batch=0, chunk=0, steps=1, images=False, passes=None, growth=2, split=None,
+              lam=0.0):
     n = sc.n
     gsb, shb = ctx.upload(sc.gs), ctx.upload(sc.sh)
     tgt = ctx.upload(np.concatenate([t.ravel() for t in sc.targets]).astype(np.float32))
@@ -45,7 +46,7 @@ def run_batch(ctx, sc, graph=True, tile_batch=0, chunk=0, steps=1, images=False,
         b.set_split(*split)
     outs = []
     for _ in range(steps):
-        b.step(gsb, shb, tgt, 0.0, grads, shape, shg, img, graph)
+        b.step(gsb, shb, tgt, lam, grads, shape, shg, img, graph)
         outs.append((ctx.download(grads, np.float32, (n, 16)), ctx.download(shape, np.float32, (n, 10)),
                      ctx.download(shg, np.float32, (n, 16, 3)),
                      ctx.download(img, np.float32, (total_px * 3,)) if images else None))
@@ -57,10 +58,13 @@ def run_batch(ctx, sc, graph=True, tile_batch=0, chunk=0, steps=1, images=False,
     return outs, entries
 
 
-def test_batch_matches_single_view_path(ctx):
+# lambda 0: the backward forms the L1 gradient itself; 0.2: the SSIM term goes
+# through the loss kernel, as on the single-view path.
+@pytest.mark.parametrize("lam", [0.0, 0.2])
+def test_batch_matches_single_view_path(ctx, lam):
     sc = small_batch_scene()
-    log(f"batch vs single: n={sc.n} views={[(c.width, c.height) for c in sc.cams]}")
-    outs, entries = run_batch(ctx, sc, graph=False, images=True)
+    log(f"batch vs single: lambda={lam} n={sc.n} views={[(c.width, c.height) for c in sc.cams]}")
+    outs, entries = run_batch(ctx, sc, graph=lam > 0, images=True, lam=lam)
     g_b, s_b, sh_b, img_b = outs[0]
     r = Renderer()
     g_sum = np.zeros((sc.n, 16))
@@ -68,7 +72,7 @@ def test_batch_matches_single_view_path(ctx):
     sh_sum = np.zeros((sc.n, 16, 3))
     imgs, ents = [], []
     for cam, tgt in zip(sc.cams, sc.targets):
-        out = r.gradients(sc.gs, cam, tgt, 0.0, sh=sc.sh, deg=3)
+        out = r.gradients(sc.gs, cam, tgt, lam, sh=sc.sh, deg=3)
         g_sum += out["grads"]
         s_sum += out["shape"]
         sh_sum += out["sh_grad"]
