@@ -356,8 +356,16 @@ __global__ __launch_bounds__(256) void bin_colsum_kernel(const ViewDesc* __restr
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T) return;
   const uint32_t* m = matrix + size_t(chunks) * vd.tile_base + t;
-  uint32_t s = 0;
-  for (uint32_t c = 0; c < chunks; ++c) s += m[size_t(c) * T];
+  // Eight rows in flight per thread: one at a time leaves the walk latency-bound.
+  uint32_t s = 0, c = 0;
+  for (; c + 8 <= chunks; c += 8) {
+    uint32_t x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = m[size_t(c + k) * T];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += x[k];
+  }
+  for (; c < chunks; ++c) s += m[size_t(c) * T];
   tile_tot[vd.tile_base + t] = s;
 }
 
@@ -410,8 +418,19 @@ __global__ __launch_bounds__(256) void bin_colscan_kernel(const ViewDesc* __rest
   if (t >= T) return;
   uint32_t* m = matrix + size_t(chunks) * vd.tile_base + t;
   uint32_t run = ranges[size_t(vd.tile_base + t) * 2];
-  for (uint32_t c = 0; c < chunks; ++c) {
-    uint32_t x = m[size_t(c) * T];
+  uint32_t c = 0;
+  for (; c + 8 <= chunks; c += 8) {  // eight rows loaded before any is rewritten
+    uint32_t x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = m[size_t(c + k) * T];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      m[size_t(c + k) * T] = run;
+      run += x[k];
+    }
+  }
+  for (; c < chunks; ++c) {
+    const uint32_t x = m[size_t(c) * T];
     m[size_t(c) * T] = run;
     run += x;
   }
