@@ -311,16 +311,28 @@ __global__ __launch_bounds__(64) void bin_count_kernel(const ViewDesc* __restric
   __syncthreads();
   const size_t seg = size_t(v) * n;
   const uint32_t begin = pbeg + c * chunk, end = min(pend, begin + chunk);
-  for (uint32_t p = begin + lane; p < end; p += 64) {
-    Rect r = unpack_rect(sorted[seg + p]);
-    uint32_t y0 = max(r.y0, bd.r0), y1 = min(r.y1, bd.r1 - 1);
-    if (r.x0 > r.x1 || y0 > y1) continue;
-    y0 -= bd.r0;
-    y1 -= bd.r0;
-    atomicAdd(&diff[y0 * W + r.x0], 1);
-    atomicAdd(&diff[y0 * W + r.x1 + 1], -1);
-    atomicAdd(&diff[(y1 + 1) * W + r.x0], -1);
-    atomicAdd(&diff[(y1 + 1) * W + r.x1 + 1], 1);
+  // Eight rectangles per lane are loaded before any is used (past the end: the
+  // empty rectangle).
+  constexpr uint32_t kAhead = 8;
+  for (uint32_t p0 = begin; p0 < end; p0 += 64 * kAhead) {
+    uint2 rc[kAhead];
+#pragma unroll
+    for (uint32_t u = 0; u < kAhead; ++u) {
+      const uint32_t p = p0 + 64 * u + lane;
+      rc[u] = p < end ? sorted[seg + p] : make_uint2(1u, 0u);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kAhead; ++u) {
+      const Rect r = unpack_rect(rc[u]);
+      uint32_t y0 = max(r.y0, bd.r0), y1 = min(r.y1, bd.r1 - 1);
+      if (r.x0 > r.x1 || y0 > y1) continue;
+      y0 -= bd.r0;
+      y1 -= bd.r0;
+      atomicAdd(&diff[y0 * W + r.x0], 1);
+      atomicAdd(&diff[y0 * W + r.x1 + 1], -1);
+      atomicAdd(&diff[(y1 + 1) * W + r.x0], -1);
+      atomicAdd(&diff[(y1 + 1) * W + r.x1 + 1], 1);
+    }
   }
   __syncthreads();
   for (uint32_t y = lane; y < rows; y += 64) {
@@ -342,8 +354,19 @@ __global__ __launch_bounds__(64) void bin_count_kernel(const ViewDesc* __restric
   const uint32_t T = tx * vd.tiles_y;
   uint32_t* out = matrix + size_t(chunks) * vd.tile_base + size_t(c) * T + size_t(bd.r0) * tx;
   const uint8_t* dn = done ? done + vd.tile_base + size_t(bd.r0) * tx : nullptr;
-  for (uint32_t k = lane; k < rows * tx; k += 64)
-    out[k] = (dn && dn[k]) ? 0u : uint32_t(diff[(k / tx) * W + k % tx]);
+  const uint32_t cells = rows * tx;
+  for (uint32_t k0 = 0; k0 < cells; k0 += 64 * kAhead) {
+    uint32_t dk[kAhead] = {};  // done flags, loaded before any count is written
+    if (dn) {
+#pragma unroll
+      for (uint32_t u = 0; u < kAhead; ++u) dk[u] = dn[min(k0 + 64 * u + lane, cells - 1)];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kAhead; ++u) {
+      const uint32_t k = k0 + 64 * u + lane;
+      if (k < cells) out[k] = dk[u] ? 0u : uint32_t(diff[(k / tx) * W + k % tx]);
+    }
+  }
 }
 
 // tile_tot[tile_base + t] = sum over chunks of the pass's counts.
@@ -510,11 +533,22 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
   const uint32_t* base = matrix + size_t(chunks) * vd.tile_base + size_t(c) * T + size_t(bd.r0) * tx;
   const uint8_t* dn = done ? done + vd.tile_base + size_t(bd.r0) * tx : nullptr;
   bool open = false;
-  for (uint32_t k = lane; k < ntile; k += 64) {
-    const bool d = dn && dn[k];
-    cursor[k] = d ? 0xFFFFFFFFu : base[k];
-    mask[k] = 0;
-    open |= !d;
+  for (uint32_t k0 = 0; k0 < ntile; k0 += 256) {  // four tiles per lane loaded at once
+    uint32_t bk[4], dk[4] = {};
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const uint32_t k = min(k0 + 64 * u + lane, ntile - 1);
+      bk[u] = base[k];
+      if (dn) dk[u] = dn[k];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const uint32_t k = k0 + 64 * u + lane;
+      if (k >= ntile) continue;
+      cursor[k] = dk[u] ? 0xFFFFFFFFu : bk[u];
+      mask[k] = 0;
+      open |= dk[u] == 0;
+    }
   }
   if (__ballot(open) == 0) return;  // every tile of the band is saturated
   const size_t seg = size_t(v) * n;

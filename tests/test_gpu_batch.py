@@ -93,6 +93,18 @@ def test_batch_matches_single_view_path(ctx, lam):
     assert rel_l2(sh_b, sh_sum) < 1e-5
 
 
+def test_batch_tuning_reads_back_the_knobs(ctx):
+    """gsr_batch_tuning returns what configure / set_split set (0 keeps a knob)."""
+    sc = small_batch_scene(n=3000, views=2, seed=4)
+    b = lib.Batch(ctx, sc.cams, sc.n, 3)
+    b.set_tuning((4, 2048, 100, 200, 300))
+    assert b.tuning() == (4, 2048, 100, 200, 300)
+    b.configure(0, 0)
+    b.set_split(0, 0, 0)
+    assert b.tuning() == (4, 2048, 0, 0, 0)
+    b.close()
+
+
 def test_batch_graph_replay_and_knobs(ctx):
     """Graph replays (several steps) agree with the eager step; tile batch and
     chunk size do not change results beyond summation order."""
