@@ -581,10 +581,12 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
       if (act) {
         const uint4 o = queue[own];
         const uint32_t k = e - o.z, ow = o.y;
-        uint32_t row = uint32_t(float(k) * (1.0f / float(ow)));
-        while (row * ow > k) --row;
-        while ((row + 1) * ow <= k) ++row;
-        t = o.x + row * tx + (k - row * ow);
+        // k < the rectangle's tiles in the band (< 2^16): an approximate
+        // reciprocal and 24-bit products, then exact correction.
+        uint32_t row = uint32_t(float(k) * __builtin_amdgcn_rcpf(float(ow)));
+        while (__umul24(row, ow) > k) --row;
+        while (__umul24(row + 1, ow) <= k) ++row;
+        t = o.x + __umul24(row, tx) + (k - __umul24(row, ow));
         idx = o.w;
         pos = cursor[t];
         atomicOr(reinterpret_cast<unsigned long long*>(&mask[t]), static_cast<unsigned long long>(1) << lane);
@@ -658,7 +660,7 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
         const uint32_t rank =
             __builtin_amdgcn_mbcnt_hi(uint32_t(hits >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(hits), 0u));
         const uint32_t w = r.x1 - r.x0 + 1;
-        queue[queued + rank] = make_uint4((y0 - bd.r0) * tx + r.x0, w, w * (y1 - y0 + 1), gi);
+        queue[queued + rank] = make_uint4(__umul24(y0 - bd.r0, tx) + r.x0, w, __umul24(w, y1 - y0 + 1), gi);
       }
       queued += uint32_t(__popcll(hits));
       if (queued >= 64) flush(64);
