@@ -15,10 +15,10 @@ def log(msg):
     print(msg, flush=True)
 
 
-def small_batch_scene(n=20000, views=5, seed=0):
+def small_batch_scene(n=20000, views=5, seed=0, sizes=None):
     sc = scenes.config(3, n=n, views=views, seed=seed)
     # shrink the views so the FP64 oracle stays fast; keep them uneven
-    sizes = [(96, 64), (160, 120), (64, 48), (200, 100), (33, 17), (128, 128), (48, 80)]
+    sizes = sizes or [(96, 64), (160, 120), (64, 48), (200, 100), (33, 17), (128, 128), (48, 80)]
     cams = []
     for v, c in enumerate(sc.cams):
         w, h = sizes[v % len(sizes)]
@@ -91,6 +91,32 @@ def test_batch_matches_single_view_path(ctx, lam):
     assert rel_l2(g_b[:, :4], g_sum[:, :4]) < 1e-5
     assert rel_l2(s_b, s_sum) < 1e-5
     assert rel_l2(sh_b, sh_sum) < 1e-5
+
+
+def test_batch_extreme_view_shapes_match_single_view_path(ctx):
+    """A 1x1 view, a 4100-px-wide strip (257 tiles: binning bands of one tile
+    row wider than the 256-tile emit budget), a 2000-px-tall strip and a view
+    one pixel past a tile edge, batched with depth slices and tile splitting:
+    the same images as view-by-view rendering, gradients to summation order."""
+    sizes = [(1, 1), (4100, 20), (20, 2000), (17, 33)]
+    sc = small_batch_scene(n=6000, views=4, seed=11, sizes=sizes)
+    log(f"batch extreme shapes: {[(c.width, c.height) for c in sc.cams]}")
+    outs, entries = run_batch(ctx, sc, graph=True, images=True, passes=2, growth=4, split=(64, 64, 64), steps=2)
+    r = Renderer()
+    g_sum = np.zeros((sc.n, 16))
+    s_sum = np.zeros((sc.n, 10))
+    imgs = []
+    for cam, tgt in zip(sc.cams, sc.targets):
+        out = r.gradients(sc.gs, cam, tgt, 0.0, sh=sc.sh, deg=3)
+        g_sum += out["grads"]
+        s_sum += out["shape"]
+        imgs.append(out["image"].ravel())
+    r.close()
+    assert entries.sum() > 0
+    for g_b, s_b, _, img_b in outs:
+        np.testing.assert_array_equal(img_b, np.concatenate(imgs))
+        assert rel_l2(g_b[:, :4], g_sum[:, :4]) < 1e-5
+        assert rel_l2(s_b, s_sum) < 1e-5
 
 
 def test_batch_tuning_reads_back_the_knobs(ctx):
