@@ -119,6 +119,32 @@ def test_batch_extreme_view_shapes_match_single_view_path(ctx):
         assert rel_l2(s_b, s_sum) < 1e-5
 
 
+@pytest.mark.parametrize("n", [0, 1, 63])
+def test_batch_tiny_gaussian_counts(ctx, n):
+    """No Gaussian, one, and fewer than a wave: the step runs (graph and eager);
+    with Gaussians it matches view-by-view rendering."""
+    sc = small_batch_scene(n=max(n, 1), views=3, seed=13)
+    sc.gs, sc.sh = sc.gs[:n], sc.sh[:n]
+    log(f"batch tiny n={n}")
+    for graph in (False, True):
+        outs, entries = run_batch(ctx, sc, graph=graph, images=True, steps=2)
+        for g_b, s_b, _, img_b in outs:
+            assert np.isfinite(img_b).all()
+            if n == 0:
+                assert not img_b.any() and int(entries.sum()) == 0
+    if n == 0:
+        return
+    r = Renderer()
+    imgs, g_sum = [], np.zeros((n, 16))
+    for cam, tgt in zip(sc.cams, sc.targets):
+        out = r.gradients(sc.gs, cam, tgt, 0.0, sh=sc.sh, deg=3)
+        imgs.append(out["image"].ravel())
+        g_sum += out["grads"]
+    r.close()
+    np.testing.assert_array_equal(outs[-1][3], np.concatenate(imgs))
+    assert np.allclose(outs[-1][0][:, :4], g_sum[:, :4], rtol=1e-5, atol=1e-12)
+
+
 def test_batch_tuning_reads_back_the_knobs(ctx):
     """gsr_batch_tuning returns what configure / set_split set (0 keeps a knob)."""
     sc = small_batch_scene(n=3000, views=2, seed=4)
