@@ -623,37 +623,73 @@ __device__ inline void window_weights(float w[kWin]) {
   }
 }
 
+// Both SSIM kernels work on 16 x 16 tiles of one view with their kWin - 1 halo
+// staged in LDS, and filter separably: the window weights are w(i) w(j), so a
+// row pass then a column pass replaces the kWin^2 taps of the direct sum by 2 kWin.
+constexpr int kSsimTile = 16;
+
+// Loads a (16 + 2R)^2 halo of `ch` interleaved channels starting at pixel
+// (x0 - R, y0 - R) into lds[ch][hy][hx]; outside the image: 0.
+template <int kR, int kCh>
+__device__ inline void load_halo(const float* __restrict__ src, int W, int H, int x0, int y0,
+                                 float (*lds)[kSsimTile + 2 * kR][kSsimTile + 2 * kR]) {
+  constexpr int kHS = kSsimTile + 2 * kR;
+  for (int k = threadIdx.x; k < kHS * kHS; k += blockDim.x) {
+    const int hy = k / kHS, hx = k % kHS;
+    const int gx = x0 + hx - kR, gy = y0 + hy - kR;
+    const bool in = gx >= 0 && gy >= 0 && gx < W && gy < H;
+    const size_t q = in ? (size_t(gy) * W + gx) * kCh : 0;
+#pragma unroll
+    for (int c = 0; c < kCh; ++c) lds[c][hy][hx] = in ? src[q + c] : 0.0f;
+  }
+}
+
 // SSIM_COEF (gpu.js:342-372) for window kWin: per valid window centre and
 // channel, the coefficients (a, b, d) with dSSIM_window/dx_p = w_p (a + 2 b x_p +
-// d y_p), from the window's weighted moments.
+// d y_p), from the window's weighted moments of x, y, x^2, y^2 and x y.
 template <int kWin>
 __global__ __launch_bounds__(256) void ssim_coef_kernel(const ViewDesc* __restrict__ views,
                                                         const float* __restrict__ image,
                                                         const float* __restrict__ target, float* __restrict__ coef) {
-  constexpr int kR = kWin / 2;
-  const ViewDesc vd = views[blockIdx.y];
-  const uint32_t P = vd.width * vd.height;
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
+  constexpr int kR = kWin / 2, kHS = kSsimTile + 2 * kR;
+  __shared__ float sx[3][kHS][kHS], sy[3][kHS][kHS];
+  __shared__ float rows[5][kHS][kSsimTile];  // row-filtered moments of one channel
+  const ViewDesc vd = views[blockIdx.z];
   const int W = int(vd.width), H = int(vd.height);
-  const int x = int(p % vd.width), y = int(p / vd.width);
-  const bool inside = x >= kR && y >= kR && x + kR < W && y + kR < H;
-  const float* img = image + vd.pixel_base * 3;
-  const float* tgt = target + vd.pixel_base * 3;
-  float* out = coef + (vd.pixel_base + p) * 9;
+  const int x0 = int(blockIdx.x) * kSsimTile, y0 = int(blockIdx.y) * kSsimTile;
+  if (x0 >= W || y0 >= H) return;  // the whole block: grid sized for the largest view
   float wt[kWin];
   window_weights<kWin>(wt);
+  load_halo<kR, 3>(image + vd.pixel_base * 3, W, H, x0, y0, sx);
+  load_halo<kR, 3>(target + vd.pixel_base * 3, W, H, x0, y0, sy);
+  __syncthreads();
+  const int ox = int(threadIdx.x) % kSsimTile, oy = int(threadIdx.x) / kSsimTile;
+  const int x = x0 + ox, y = y0 + oy;
+  const bool pixel = x < W && y < H;
+  const bool centre = x >= kR && y >= kR && x + kR < W && y + kR < H;
+  float* out = coef + (vd.pixel_base + size_t(y) * W + x) * 9;
   for (int c = 0; c < 3; ++c) {
+    for (int k = threadIdx.x; k < kHS * kSsimTile; k += blockDim.x) {
+      const int hy = k / kSsimTile, hx = k % kSsimTile;
+      float m[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int d = 0; d < kWin; ++d) {
+        const float xv = sx[c][hy][hx + d], yv = sy[c][hy][hx + d], w = wt[d];
+        m[0] += w * xv; m[1] += w * yv; m[2] += w * xv * xv; m[3] += w * yv * yv; m[4] += w * xv * yv;
+      }
+#pragma unroll
+      for (int q = 0; q < 5; ++q) rows[q][hy][hx] = m[q];
+    }
+    __syncthreads();
+    float mx = 0.0f, my = 0.0f, sxx = 0.0f, syy = 0.0f, sxy = 0.0f;
+#pragma unroll
+    for (int d = 0; d < kWin; ++d) {
+      const float w = wt[d];
+      mx += w * rows[0][oy + d][ox]; my += w * rows[1][oy + d][ox];
+      sxx += w * rows[2][oy + d][ox]; syy += w * rows[3][oy + d][ox]; sxy += w * rows[4][oy + d][ox];
+    }
     float ca = 0.0f, cb = 0.0f, cd = 0.0f;
-    if (inside) {
-      float mx = 0.0f, my = 0.0f, sxx = 0.0f, syy = 0.0f, sxy = 0.0f;
-      for (int dy = 0; dy < kWin; ++dy)
-        for (int dx = 0; dx < kWin; ++dx) {
-          size_t q = (size_t(y + dy - kR) * W + x + dx - kR) * 3 + c;
-          const float w = wt[dy] * wt[dx];
-          float xv = img[q], yv = tgt[q];
-          mx += w * xv; my += w * yv; sxx += w * xv * xv; syy += w * yv * yv; sxy += w * xv * yv;
-        }
+    if (centre) {
       const float c1 = 0.0001f, c2 = 0.0009f;
       float a1 = 2.0f * mx * my + c1;
       float a2 = 2.0f * (sxy - mx * my) + c2;
@@ -664,49 +700,85 @@ __global__ __launch_bounds__(256) void ssim_coef_kernel(const ViewDesc* __restri
       cb = -ssim / b2;
       cd = 2.0f * ssim / a2;
     }
-    out[c * 3] = ca;
-    out[c * 3 + 1] = cb;
-    out[c * 3 + 2] = cd;
+    if (pixel) {
+      out[c * 3] = ca;
+      out[c * 3 + 1] = cb;
+      out[c * 3 + 2] = cd;
+    }
+    __syncthreads();  // rows is refilled for the next channel
   }
 }
 
+// LOSS_GRAD (gpu.js:383-409) without the SSIM term: (1 - lambda) sign(x - y) /
+// (3P), one thread per pixel.
+__global__ __launch_bounds__(256) void loss_l1_kernel(const ViewDesc* __restrict__ views,
+                                                      const float* __restrict__ image,
+                                                      const float* __restrict__ target, float lambda,
+                                                      float* __restrict__ grad) {
+  const ViewDesc vd = views[blockIdx.y];
+  const uint32_t P = vd.width * vd.height;
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const float n = float(P * 3u);
+  const size_t pix = vd.pixel_base + p;
+  for (int c = 0; c < 3; ++c)
+    grad[pix * 3 + c] = (1.0f - lambda) * sign_f(image[pix * 3 + c] - target[pix * 3 + c]) / n;
+}
 
 // LOSS_GRAD (gpu.js:383-409): dLoss/dpixel of (1 - lambda) L1 + lambda (1 - SSIM)
 // = (1 - lambda) sign(x - y) / (3P) - lambda / windows * sum over the windows
-// holding the pixel of w (a + 2 b x + d y).
+// holding the pixel of w (a + 2 b x + d y).  The window weights are symmetric,
+// so the sum over the windows holding a pixel is the same separable filter over
+// the coefficient maps (zero where no window is centred).
 template <int kWin>
 __global__ __launch_bounds__(256) void loss_grad_kernel(const ViewDesc* __restrict__ views,
                                                         const float* __restrict__ image,
                                                         const float* __restrict__ target,
                                                         const float* __restrict__ coef, float lambda,
                                                         float* __restrict__ grad) {
-  constexpr int kR = kWin / 2;
-  const ViewDesc vd = views[blockIdx.y];
-  const uint32_t P = vd.width * vd.height;
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
+  constexpr int kR = kWin / 2, kHS = kSsimTile + 2 * kR;
+  __shared__ float sc[9][kHS][kHS];
+  __shared__ float rows[9][kHS][kSsimTile];
+  const ViewDesc vd = views[blockIdx.z];
   const int W = int(vd.width), H = int(vd.height);
-  const int x0 = int(p % vd.width), y0 = int(p / vd.width);
-  const float n = float(P * 3u);
+  const int x0 = int(blockIdx.x) * kSsimTile, y0 = int(blockIdx.y) * kSsimTile;
+  if (x0 >= W || y0 >= H) return;  // the whole block
   const float windows = float(max(0, W - 2 * kR) * max(0, H - 2 * kR) * 3);
-  const size_t pix = vd.pixel_base + p;
   float wt[kWin];
-  if (lambda > 0.0f) window_weights<kWin>(wt);
+  window_weights<kWin>(wt);
+  if (windows > 0.0f) {
+    load_halo<kR, 9>(coef + vd.pixel_base * 9, W, H, x0, y0, sc);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kHS * kSsimTile; k += blockDim.x) {
+      const int hy = k / kSsimTile, hx = k % kSsimTile;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int d = 0; d < kWin; ++d) acc += wt[d] * sc[q][hy][hx + d];
+        rows[q][hy][hx] = acc;
+      }
+    }
+    __syncthreads();
+  }
+  const int ox = int(threadIdx.x) % kSsimTile, oy = int(threadIdx.x) / kSsimTile;
+  const int x = x0 + ox, y = y0 + oy;
+  if (x >= W || y >= H) return;
+  const float n = float(uint32_t(W * H) * 3u);
+  const size_t pix = vd.pixel_base + size_t(y) * W + x;
   for (int c = 0; c < 3; ++c) {
-    float xv = image[pix * 3 + c], yv = target[pix * 3 + c];
+    const float xv = image[pix * 3 + c], yv = target[pix * 3 + c];
     float g = (1.0f - lambda) * sign_f(xv - yv) / n;
-    if (lambda > 0.0f && windows > 0.0f) {
+    if (windows > 0.0f) {
       float sa = 0.0f, sb = 0.0f, sd = 0.0f;
-      for (int dy = -kR; dy <= kR; ++dy)
-        for (int dx = -kR; dx <= kR; ++dx) {
-          int cx = x0 + dx, cy = y0 + dy;
-          if (cx < 0 || cy < 0 || cx >= W || cy >= H) continue;
-          const float* k = coef + (vd.pixel_base + size_t(cy) * W + cx) * 9 + c * 3;
-          // the pixel sits at offset (-dx, -dy) of the window centred at (cx, cy)
-          const float w = wt[kR - dy] * wt[kR - dx];
-          sa += w * k[0]; sb += w * k[1]; sd += w * k[2];
-        }
-      float dS = (sa + 2.0f * xv * sb + yv * sd) / windows;
+#pragma unroll
+      for (int d = 0; d < kWin; ++d) {
+        const float w = wt[d];
+        sa += w * rows[c * 3][oy + d][ox];
+        sb += w * rows[c * 3 + 1][oy + d][ox];
+        sd += w * rows[c * 3 + 2][oy + d][ox];
+      }
+      const float dS = (sa + 2.0f * xv * sb + yv * sd) / windows;
       g -= lambda * dS;
     }
     grad[pix * 3 + c] = g;
@@ -851,13 +923,23 @@ void launch_loss_grad(hipStream_t s, const ViewDesc* views, const ViewDesc* host
     maxp = p > maxp ? p : maxp;
   }
   if (maxp == 0) return;
-  dim3 grid((maxp + 255) / 256, num_views);
+  if (!(lambda > 0.0f)) {
+    hipLaunchKernelGGL(loss_l1_kernel, dim3((maxp + 255) / 256, num_views), dim3(256), 0, s, views, image, target,
+                       lambda, grad);
+    return;
+  }
+  uint32_t maxw = 0, maxh = 0;
+  for (uint32_t v = 0; v < num_views; ++v) {
+    maxw = std::max(maxw, host_views[v].width);
+    maxh = std::max(maxh, host_views[v].height);
+  }
+  const dim3 tiles((maxw + kSsimTile - 1) / kSsimTile, (maxh + kSsimTile - 1) / kSsimTile, num_views);
   if (window == 11) {
-    if (lambda > 0.0f) hipLaunchKernelGGL(ssim_coef_kernel<11>, grid, dim3(256), 0, s, views, image, target, coef);
-    hipLaunchKernelGGL(loss_grad_kernel<11>, grid, dim3(256), 0, s, views, image, target, coef, lambda, grad);
+    hipLaunchKernelGGL(ssim_coef_kernel<11>, tiles, dim3(256), 0, s, views, image, target, coef);
+    hipLaunchKernelGGL(loss_grad_kernel<11>, tiles, dim3(256), 0, s, views, image, target, coef, lambda, grad);
   } else {
-    if (lambda > 0.0f) hipLaunchKernelGGL(ssim_coef_kernel<7>, grid, dim3(256), 0, s, views, image, target, coef);
-    hipLaunchKernelGGL(loss_grad_kernel<7>, grid, dim3(256), 0, s, views, image, target, coef, lambda, grad);
+    hipLaunchKernelGGL(ssim_coef_kernel<7>, tiles, dim3(256), 0, s, views, image, target, coef);
+    hipLaunchKernelGGL(loss_grad_kernel<7>, tiles, dim3(256), 0, s, views, image, target, coef, lambda, grad);
   }
 }
 

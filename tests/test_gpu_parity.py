@@ -231,7 +231,10 @@ def test_loss_grad_matches_oracle(lam):
     r.close()
 
 
-@pytest.mark.parametrize("window,size", [(7, (37, 23)), (11, (37, 23)), (11, (130, 70))])
+# sizes: one tile, several tiles with a ragged edge, and smaller than the window
+# in one direction (no window: the L1 term alone)
+@pytest.mark.parametrize("window,size", [(7, (37, 23)), (11, (37, 23)), (11, (130, 70)), (7, (300, 6)),
+                                         (11, (10, 40))])
 def test_loss_grad_window_matches_oracle(window, size):
     W, H = size
     image, target = lcg_images(W, H)
