@@ -97,7 +97,16 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   __shared__ uint32_t tmp[kSortThreads];
   __shared__ uint32_t lkeys[kSortChunk];
   __shared__ uint32_t lvals[kSortChunk];
-  const uint32_t v = blockIdx.y, c = blockIdx.x;
+  // The gathering pass reads rects[v] at random: when the views split evenly
+  // over the eight dies (workgroup L runs on die L mod 8, each die with its own
+  // L2), die d takes views d, d + 8, ... one after another, so its L2 holds one
+  // view's rectangles rather than a slice of every view's.
+  uint32_t v = blockIdx.y, c = blockIdx.x;
+  if (rect_sorted && gridDim.y % 8 == 0) {
+    const uint32_t L = blockIdx.x + gridDim.x * blockIdx.y, k = L / 8;
+    v = L % 8 + 8 * (k / gridDim.x);
+    c = k % gridDim.x;
+  }
   const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
 #pragma unroll
   for (int k = 0; k < 4; ++k) wcnt[k][threadIdx.x] = 0;
