@@ -289,10 +289,11 @@ int gsr_bin_sort(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, uint3
       (rc = pooled_t(ctx, "overflow", 1, &overflow)))
     return rc;
   launch_zero(ctx->stream, overflow, sizeof(uint32_t));
-  launch_prep(ctx->stream, dview, 1, splats, n, keys, rects);
+  const bool packed = rect_packed(std::max(host.tiles_x, host.tiles_y));
+  launch_prep(ctx->stream, dview, 1, splats, n, keys, rects, packed);
   SortScratch sc{keys_alt, vals, vals_alt, hist, piece};
-  launch_depth_sort(ctx->stream, keys, n, 1, sc, rects, rect_sorted);
-  BinScratch bs{matrix, tile_tot, rect_sorted, run};
+  launch_depth_sort(ctx->stream, keys, n, 1, sc, rects, rect_sorted, packed);
+  BinScratch bs{matrix, tile_tot, rect_sorted, run, packed};
   BinPass pass{0, n, nullptr, ranges, ebase, entries, true};
   launch_bin_count(ctx->stream, dview, &host, 1, n, chunk, bs, pass);
   GSR_LAUNCHED(ctx);

@@ -234,10 +234,14 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   launch_zero(s, b->walk, size_t(b->num_passes) * b->total_tiles * sizeof(uint32_t));
   if (prof && prof->last) launch_zero(s, prof->last, size_t(b->num_passes) * b->total_tiles * sizeof(uint32_t));
   mark();
-  launch_project(s, b->d_cams, b->d_views, b->V, gs, sh, b->deg, b->n, b->splats, b->keys, b->rects, nullptr);
+  uint32_t axis = 0;
+  for (const ViewDesc& vd : b->views) axis = std::max(axis, std::max(vd.tiles_x, vd.tiles_y));
+  const bool packed = rect_packed(axis);
+  launch_project(s, b->d_cams, b->d_views, b->V, gs, sh, b->deg, b->n, b->splats, b->keys, b->rects, nullptr,
+                 packed);
   mark();
   SortScratch sc{b->keys_alt, b->vals, b->vals_alt, b->hist, b->piece};
-  launch_depth_sort(s, b->keys, b->n, b->V, sc, b->rects, b->rect_sorted);
+  launch_depth_sort(s, b->keys, b->n, b->V, sc, b->rects, b->rect_sorted, packed);
   mark();
   RasterArgs a{};
   a.views = b->d_views;
@@ -264,7 +268,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   // Binning and forward alternate per depth slice: a slice is binned only for
   // the tiles the slices before it left unsaturated.  Profiled steps time all
   // binning as "bin count" + "bin emit" and all forward passes as "forward".
-  BinScratch bs{b->matrix, b->tile_tot, b->rect_sorted, b->run};
+  BinScratch bs{b->matrix, b->tile_tot, b->rect_sorted, b->run, packed};
   float t_count = 0.f, t_emit = 0.f, t_fwd = 0.f;
   hipEvent_t e0, e1, e2, e3;
   if (prof) {

@@ -52,13 +52,20 @@ __host__ __device__ inline uint32_t work_last(const TileWork& w) { return (w.vie
 // ---- launch helpers (host side), implemented per translation unit -------------
 
 // Projection (gsr_project.hip).  cams: device array of num_views cameras.
+// Tile rectangles of the binning, x0 x1 y0 y1 inclusive (empty: x0 > x1): in a
+// [V][n] buffer of uint2 (16 bits per coordinate), or packed into its first
+// [V][n] uint32 words (8 bits per coordinate) when no view has more than
+// kPackedRectTiles tiles along an axis (rect_packed), which halves the rectangle
+// traffic of the sort and the binning.
+constexpr uint32_t kPackedRectTiles = 256;
+inline bool rect_packed(uint32_t max_tiles_axis) { return max_tiles_axis <= kPackedRectTiles; }
 // splats [V][n][12]; keys/rects/counts optional [V][n] (binning inputs).
 void launch_project(hipStream_t s, const gsr_camera* cams, const ViewDesc* views, uint32_t num_views,
                     const float* gs, const float* sh, int sh_degree, uint32_t n, float* splats, uint32_t* keys,
-                    uint2* rects, uint32_t* counts);
+                    uint2* rects, uint32_t* counts, bool packed = false);
 // Binning inputs from existing splat records (drop-in path).
 void launch_prep(hipStream_t s, const ViewDesc* views, uint32_t num_views, const float* splats, uint32_t n,
-                 uint32_t* keys, uint2* rects);
+                 uint32_t* keys, uint2* rects, bool packed);
 
 // Sorting and binning (gsr_sort.hip).
 struct SortScratch {
@@ -74,13 +81,14 @@ size_t scan_piece_elems(size_t seg_len, uint32_t num_segs);
 // rect_sorted[v][p] = rects[v][vals[v][p]] (the sorted keys themselves are not
 // kept: keys is scratch afterwards).  Four 8-bit passes.
 void launch_depth_sort(hipStream_t s, uint32_t* keys, uint32_t n, uint32_t num_views, const SortScratch& sc,
-                       const uint2* rects, uint2* rect_sorted);
+                       const uint2* rects, uint2* rect_sorted, bool packed);
 
 struct BinScratch {
   uint32_t* matrix;      // counts of one pass: [view][chunk][tile] at chunks * tile_base
   uint32_t* tile_tot;    // [total tiles]
   uint2* rect_sorted;    // [V][n] tile rectangles in depth order
   uint64_t* run;         // device scalar: order slots used by the passes so far
+  bool packed;           // rectangle format (rect_packed)
 };
 // One binning pass: the depth slice [pbeg, pend) of every view's sorted
 // Gaussians.  Its entries go to order[ebase[v] + ranges[...]].

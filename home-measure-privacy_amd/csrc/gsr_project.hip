@@ -58,14 +58,18 @@ __device__ inline void write_splat(float* __restrict__ splats, size_t idx, const
 
 // Depth key and tile rectangle of half-widths (rx, ry) around (u, v); splats
 // without tiles get the invalid key.
-__device__ inline void write_bin_inputs(uint32_t* keys, uint2* rects, uint32_t* counts, size_t idx, bool valid,
+__device__ inline void write_bin_inputs(uint32_t* keys, uint2* rects, bool packed, uint32_t* counts, size_t idx,
+                                        bool valid,
                                         float u, float v, float rx, float ry, float depth, uint32_t tiles_x,
                                         uint32_t tiles_y) {
   int x0 = 0, x1 = -1, y0 = 0, y1 = -1;
   bool any = valid && tile_rect(u, v, rx, ry, tiles_x, tiles_y, x0, x1, y0, y1);
   uint32_t cnt = any ? uint32_t(x1 - x0 + 1) * uint32_t(y1 - y0 + 1) : 0u;
   if (keys) keys[idx] = any ? __float_as_uint(depth) : kInvalidKey;
-  if (rects)
+  if (rects && packed)
+    reinterpret_cast<uint32_t*>(rects)[idx] =
+        any ? uint32_t(x0) | (uint32_t(x1) << 8) | (uint32_t(y0) << 16) | (uint32_t(y1) << 24) : 1u;
+  else if (rects)
     rects[idx] = any ? make_uint2(uint32_t(x0) | (uint32_t(x1) << 16), uint32_t(y0) | (uint32_t(y1) << 16))
                      : make_uint2(1u, 0u);
   if (counts) counts[idx] = cnt;
@@ -76,7 +80,7 @@ __global__ __launch_bounds__(256) void project_kernel(const gsr_camera* __restri
                                                       const float* __restrict__ gs, const float* __restrict__ sh,
                                                       int deg, uint32_t n, float* __restrict__ splats,
                                                       uint32_t* __restrict__ keys, uint2* __restrict__ rects,
-                                                      uint32_t* __restrict__ counts) {
+                                                      uint32_t* __restrict__ counts, bool packed) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   GaussLocal g;
@@ -99,7 +103,7 @@ __global__ __launch_bounds__(256) void project_kernel(const gsr_camera* __restri
         rx = e.x < 0.0f ? e.x : fminf(p.radius, e.x + 0.5f);
         ry = e.y < 0.0f ? e.y : fminf(p.radius, e.y + 0.5f);
       }
-      write_bin_inputs(keys, rects, counts, idx, p.ok, p.u, p.v, rx, ry, p.depth, views[v].tiles_x,
+      write_bin_inputs(keys, rects, packed, counts, idx, p.ok, p.u, p.v, rx, ry, p.depth, views[v].tiles_x,
                        views[v].tiles_y);
     }
   }
@@ -109,7 +113,8 @@ __global__ __launch_bounds__(256) void project_kernel(const gsr_camera* __restri
 // (gpu.js:454), u, v, radius, depth from the record.
 __global__ __launch_bounds__(256) void prep_kernel(const ViewDesc* __restrict__ views, uint32_t num_views,
                                                    const float* __restrict__ splats, uint32_t n,
-                                                   uint32_t* __restrict__ keys, uint2* __restrict__ rects) {
+                                                   uint32_t* __restrict__ keys, uint2* __restrict__ rects,
+                                                   bool packed) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t v = blockIdx.y;
   if (i >= n || v >= num_views) return;
@@ -117,25 +122,26 @@ __global__ __launch_bounds__(256) void prep_kernel(const ViewDesc* __restrict__ 
   const float4* r4 = reinterpret_cast<const float4*>(splats + idx * 12);
   float4 a = r4[0], b = r4[1], c = r4[2];
   bool valid = c.w > 0.5f;
-  write_bin_inputs(keys, rects, nullptr, idx, valid, a.x, a.y, b.z, b.z, b.w, views[v].tiles_x, views[v].tiles_y);
+  write_bin_inputs(keys, rects, packed, nullptr, idx, valid, a.x, a.y, b.z, b.z, b.w, views[v].tiles_x,
+                   views[v].tiles_y);
 }
 
 }  // namespace
 
 void launch_project(hipStream_t s, const gsr_camera* cams, const ViewDesc* views, uint32_t num_views,
                     const float* gs, const float* sh, int sh_degree, uint32_t n, float* splats, uint32_t* keys,
-                    uint2* rects, uint32_t* counts) {
+                    uint2* rects, uint32_t* counts, bool packed) {
   if (n == 0 || num_views == 0) return;
   dim3 grid((n + 255) / 256);
   hipLaunchKernelGGL(project_kernel, grid, dim3(256), 0, s, cams, views, num_views, gs, sh, sh_degree, n, splats,
-                     keys, rects, counts);
+                     keys, rects, counts, packed);
 }
 
 void launch_prep(hipStream_t s, const ViewDesc* views, uint32_t num_views, const float* splats, uint32_t n,
-                 uint32_t* keys, uint2* rects) {
+                 uint32_t* keys, uint2* rects, bool packed) {
   if (n == 0 || num_views == 0) return;
   dim3 grid((n + 255) / 256, num_views);
-  hipLaunchKernelGGL(prep_kernel, grid, dim3(256), 0, s, views, num_views, splats, n, keys, rects);
+  hipLaunchKernelGGL(prep_kernel, grid, dim3(256), 0, s, views, num_views, splats, n, keys, rects, packed);
 }
 
 }  // namespace gsr

@@ -90,7 +90,7 @@ __device__ inline uint32_t block_exclusive_scan256(uint32_t x, uint32_t* tmp);
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t n, uint32_t chunks,
     int shift, const uint32_t* __restrict__ hist, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
-    const uint2* __restrict__ rects, uint2* __restrict__ rect_sorted) {
+    const uint2* __restrict__ rects, uint2* __restrict__ rect_sorted, bool packed) {
   __shared__ uint32_t wcnt[4][kBins];  // per-wave digit counts, then the waves' local digit bases
   __shared__ uint32_t lstart[kBins];   // first local slot of each digit
   __shared__ uint32_t gpos[kBins];     // first output slot of each digit for this chunk
@@ -166,7 +166,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t dst = gpos[d] + (li - lstart[d]);
     const uint32_t val = lvals[li];
     vals_out[seg + dst] = val;
-    if (rect_sorted)
+    if (rect_sorted && packed)
+      reinterpret_cast<uint32_t*>(rect_sorted)[seg + dst] = reinterpret_cast<const uint32_t*>(rects)[seg + val];
+    else if (rect_sorted)
       rect_sorted[seg + dst] = rects[seg + val];
     else
       keys_out[seg + dst] = key;
@@ -283,9 +285,18 @@ struct Rect {
   uint32_t x0, x1, y0, y1;  // inclusive; empty when x0 > x1 or y0 > y1
 };
 
-__device__ inline Rect unpack_rect(uint2 rc) {
-  return Rect{rc.x & 0xFFFFu, rc.x >> 16, rc.y & 0xFFFFu, rc.y >> 16};
+// A rectangle as loaded (raw.x holds the packed word when kPacked).
+template <bool kPacked>
+__device__ inline Rect unpack_rect(uint2 raw) {
+  if (kPacked) return Rect{raw.x & 0xFFu, (raw.x >> 8) & 0xFFu, (raw.x >> 16) & 0xFFu, raw.x >> 24};
+  return Rect{raw.x & 0xFFFFu, raw.x >> 16, raw.y & 0xFFFFu, raw.y >> 16};
 }
+template <bool kPacked>
+__device__ inline uint2 load_rect(const uint2* __restrict__ rects, size_t i) {
+  if (kPacked) return make_uint2(reinterpret_cast<const uint32_t*>(rects)[i], 0u);
+  return rects[i];
+}
+__device__ inline uint2 empty_rect() { return make_uint2(1u, 0u); }  // x0 = 1 > x1 = 0 in both forms
 
 struct Band {
   uint32_t tx, r0, r1;  // tiles per row, band rows [r0, r1)
@@ -305,6 +316,7 @@ __device__ inline Band band_of(const ViewDesc& vd, uint32_t band, uint32_t budge
 // Pass 1: per (chunk, tile) entry counts.  Each Gaussian adds its rectangle
 // (clipped to the band) to a 2D difference array with four LDS atomics; prefix
 // sums along x then y turn it into counts.  Done tiles count zero.
+template <bool kPacked>
 __global__ __launch_bounds__(64) void bin_count_kernel(const ViewDesc* __restrict__ views, uint32_t n, uint32_t chunk,
                                                        uint32_t pbeg, uint32_t pend, uint32_t chunks,
                                                        const uint2* __restrict__ sorted,
@@ -328,11 +340,11 @@ __global__ __launch_bounds__(64) void bin_count_kernel(const ViewDesc* __restric
 #pragma unroll
     for (uint32_t u = 0; u < kAhead; ++u) {
       const uint32_t p = p0 + 64 * u + lane;
-      rc[u] = p < end ? sorted[seg + p] : make_uint2(1u, 0u);
+      rc[u] = p < end ? load_rect<kPacked>(sorted, seg + p) : empty_rect();
     }
 #pragma unroll
     for (uint32_t u = 0; u < kAhead; ++u) {
-      const Rect r = unpack_rect(rc[u]);
+      const Rect r = unpack_rect<kPacked>(rc[u]);
       uint32_t y0 = max(r.y0, bd.r0), y1 = min(r.y1, bd.r1 - 1);
       if (r.x0 > r.x1 || y0 > y1) continue;
       y0 -= bd.r0;
@@ -515,6 +527,7 @@ __device__ inline uint32_t emit_block(uint32_t L, uint32_t total) {
   return L / group * group + (L % 8) * kEmitDieRun + (L / 8) % kEmitDieRun;
 }
 
+template <bool kPacked>
 __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict__ views, uint32_t n, uint32_t chunk,
                                                       uint32_t pbeg, uint32_t pend, uint32_t chunks,
                                                       const uint32_t* __restrict__ vals,
@@ -634,10 +647,10 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
 #pragma unroll
     for (int u = 0; u < kAhead; ++u) {
       const uint32_t p = q0 + 64 * u + lane;
-      nrc[u] = make_uint2(1u, 0u);  // empty rectangle
+      nrc[u] = empty_rect();
       ngi[u] = 0;
       if (p < end) {
-        nrc[u] = sorted[seg + p];
+        nrc[u] = load_rect<kPacked>(sorted, seg + p);
         ngi[u] = vals[seg + p];
       }
     }
@@ -654,7 +667,7 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
     if (q0 + 64 * kAhead < end) load_round(q0 + 64 * kAhead);
 #pragma unroll 1
     for (int u = 0; u < kAhead; ++u) {
-      const Rect r = unpack_rect(crc[0]);
+      const Rect r = unpack_rect<kPacked>(crc[0]);
       const uint32_t gi = cgi[0];
 #pragma unroll
       for (int k = 0; k + 1 < kAhead; ++k) {
@@ -691,7 +704,7 @@ size_t scan_piece_elems(size_t seg_len, uint32_t num_segs) {
 }
 
 void launch_depth_sort(hipStream_t s, uint32_t* keys, uint32_t n, uint32_t num_views, const SortScratch& sc,
-                       const uint2* rects, uint2* rect_sorted) {
+                       const uint2* rects, uint2* rect_sorted, bool packed) {
   if (n == 0 || num_views == 0) return;
   const uint32_t chunks = (n + kSortChunk - 1) / kSortChunk;
   dim3 grid(chunks, num_views);
@@ -709,7 +722,7 @@ void launch_depth_sort(hipStream_t s, uint32_t* keys, uint32_t n, uint32_t num_v
       hipLaunchKernelGGL(radix_hist_kernel<false>, grid, dim3(kSortThreads), 0, s, kin, n, chunks, shift, sc.hist);
     scan_segments(s, sc.hist, size_t(kBins) * chunks, num_views, sc.piece);
     hipLaunchKernelGGL(radix_scatter_kernel, grid, dim3(kSortThreads), 0, s, kin, vin, n, chunks, shift, sc.hist,
-                       kout, vout, rects, last ? rect_sorted : nullptr);
+                       kout, vout, rects, last ? rect_sorted : nullptr, packed);
     // Ping-pong: after an even number of passes the values are in `vals`.
     uint32_t* kt = kin; kin = kout; kout = kt;
     vin = vout;
@@ -755,8 +768,13 @@ void launch_bin_count(hipStream_t s, const ViewDesc* views, const ViewDesc* host
   const uint32_t chunks = slice_chunks(pass, chunk);
   const BandGeometry g = band_geometry(host_views, num_views);
   if (chunks > 0) {
-    hipLaunchKernelGGL(bin_count_kernel, dim3(chunks, g.max_bands, num_views), dim3(64), g.count_lds, s, views, n,
-                       chunk, pass.pbeg, pass.pend, chunks, bs.rect_sorted, pass.done, bs.matrix);
+    const dim3 grid(chunks, g.max_bands, num_views);
+    if (bs.packed)
+      hipLaunchKernelGGL(bin_count_kernel<true>, grid, dim3(64), g.count_lds, s, views, n, chunk, pass.pbeg,
+                         pass.pend, chunks, bs.rect_sorted, pass.done, bs.matrix);
+    else
+      hipLaunchKernelGGL(bin_count_kernel<false>, grid, dim3(64), g.count_lds, s, views, n, chunk, pass.pbeg,
+                         pass.pend, chunks, bs.rect_sorted, pass.done, bs.matrix);
     dim3 tgrid((g.max_tiles + 255) / 256, num_views);
     hipLaunchKernelGGL(bin_colsum_kernel, tgrid, dim3(256), 0, s, views, chunks, bs.matrix, bs.tile_tot);
   }
@@ -774,10 +792,15 @@ void launch_bin_emit(hipStream_t s, const ViewDesc* views, const ViewDesc* host_
   const BandGeometry g = band_geometry(host_views, num_views);
   dim3 tgrid((g.max_tiles + 255) / 256, num_views);
   hipLaunchKernelGGL(bin_colscan_kernel, tgrid, dim3(256), 0, s, views, chunks, pass.ranges, bs.matrix);
-  hipLaunchKernelGGL(bin_emit_kernel, dim3(chunks, g.max_emit_bands, num_views), dim3(64), g.emit_lds, s, views, n,
-                     chunk,
-                     pass.pbeg, pass.pend, chunks, vals, bs.rect_sorted, pass.done, bs.matrix, pass.ebase, order,
-                     order_capacity, overflow);
+  const dim3 grid(chunks, g.max_emit_bands, num_views);
+  if (bs.packed)
+    hipLaunchKernelGGL(bin_emit_kernel<true>, grid, dim3(64), g.emit_lds, s, views, n, chunk, pass.pbeg, pass.pend,
+                       chunks, vals, bs.rect_sorted, pass.done, bs.matrix, pass.ebase, order, order_capacity,
+                       overflow);
+  else
+    hipLaunchKernelGGL(bin_emit_kernel<false>, grid, dim3(64), g.emit_lds, s, views, n, chunk, pass.pbeg, pass.pend,
+                       chunks, vals, bs.rect_sorted, pass.done, bs.matrix, pass.ebase, order, order_capacity,
+                       overflow);
 }
 
 }  // namespace gsr
