@@ -658,6 +658,18 @@ int gsr_batch_tuning(gsr_batch* b, uint32_t* tuning) {
 int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const float* targets, float lambda,
                        float* grads, float* shape, float* sh_grad, uint32_t* tile_batch, uint32_t* chunk) {
   if (!b) return GSR_E_INVALID;
+  // The shortest of `reps` single steps (each returns after its stream is done):
+  // the minimum is steadier than a mean against the host's own hiccups.
+  auto time_min = [&](int reps, double& dt) {
+    dt = 1e30;
+    for (int k = 0; k < reps; ++k) {
+      const auto t0 = std::chrono::steady_clock::now();
+      const int rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
+      if (rc) return rc;
+      dt = std::min(dt, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    }
+    return int(GSR_OK);
+  };
   const uint32_t tbs[] = {1, 2, 4, 8};
   const uint32_t chunks[] = {1024, 2048, 4096, 8192};
   double best = 1e30;
@@ -666,15 +678,11 @@ int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const flo
     for (uint32_t tb : tbs) {
       int rc = gsr_batch_configure(b, tb, ch);
       if (rc) return rc;
-      // Warm (captures the graph), then time two replays.
+      // Warm (captures the graph), then time replays.
       rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
       if (rc) return rc;
-      auto t0 = std::chrono::steady_clock::now();
-      for (int k = 0; k < 2; ++k) {
-        rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
-        if (rc) return rc;
-      }
-      double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      double dt;
+      if ((rc = time_min(4, dt))) return rc;
       if (dt < best) {
         best = dt;
         best_tb = tb;
@@ -696,12 +704,8 @@ int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const flo
     drop_graph(b);  // the threshold is a launch argument of the captured step
     rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
     if (rc) return rc;
-    auto t0 = std::chrono::steady_clock::now();
-    for (int k = 0; k < 3; ++k) {
-      rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
-      if (rc) return rc;
-    }
-    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    double dt;
+    if ((rc = time_min(4, dt))) return rc;
     if (dt < best_split_t) {
       best_split_t = dt;
       best_split = sp;
@@ -720,12 +724,8 @@ int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const flo
     drop_graph(b);
     rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
     if (rc) return rc;
-    auto t0 = std::chrono::steady_clock::now();
-    for (int k = 0; k < 3; ++k) {
-      rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
-      if (rc) return rc;
-    }
-    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    double dt;
+    if ((rc = time_min(4, dt))) return rc;
     if (dt < best_later_t) {
       best_later_t = dt;
       best_later = sl;
@@ -745,12 +745,8 @@ int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const flo
       rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
       if (rc) return rc;
     }
-    auto t0 = std::chrono::steady_clock::now();
-    for (int k = 0; k < 3; ++k) {
-      rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
-      if (rc) return rc;
-    }
-    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    double dt;
+    if ((rc = time_min(4, dt))) return rc;
     if (dt < best_first_t) {
       best_first_t = dt;
       best_first = sf;
