@@ -251,8 +251,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--tune", default="", help="tile_batch,chunk,split_first,split_later,split_backward to set "
-                    "instead of autotuning (the 'tuning' line of an earlier run)")
+    ap.add_argument("--tune", default="", help="tile_batch,chunk,split_first,split_later,split_backward[,emit_band] "
+                    "to set instead of autotuning (the 'tuning' line of an earlier run)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0, help="only time a device copy of this many MiB")
     ap.add_argument("--train", action="store_true", help="the 3M-Gaussian, 64-view training loop instead")

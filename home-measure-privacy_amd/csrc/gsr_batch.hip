@@ -78,6 +78,7 @@ struct gsr_batch {
   // length says little about the walk), the forward of later slices (unsaturated
   // tiles walk their whole segment) and the backward (entries the forward walked).
   uint32_t split_first = 0, split_later = 0, split_backward = 1024;
+  uint32_t emit_band = 256;  // tiles per band of the emit kernel
   uint32_t* order = nullptr;
   uint64_t order_cap = 0;
   // [0] overflow, [16 + p] split-item count of forward pass p, [16 + kMaxPasses]
@@ -268,7 +269,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   // Binning and forward alternate per depth slice: a slice is binned only for
   // the tiles the slices before it left unsaturated.  Profiled steps time all
   // binning as "bin count" + "bin emit" and all forward passes as "forward".
-  BinScratch bs{b->matrix, b->tile_tot, b->rect_sorted, b->run, packed};
+  BinScratch bs{b->matrix, b->tile_tot, b->rect_sorted, b->run, packed, b->emit_band};
   float t_count = 0.f, t_emit = 0.f, t_fwd = 0.f;
   hipEvent_t e0, e1, e2, e3;
   if (prof) {
@@ -652,6 +653,17 @@ int gsr_batch_tuning(gsr_batch* b, uint32_t* tuning) {
   tuning[2] = b->split_first;
   tuning[3] = b->split_later;
   tuning[4] = b->split_backward;
+  tuning[5] = b->emit_band;
+  return GSR_OK;
+}
+
+int gsr_batch_set_emit_band(gsr_batch* b, uint32_t tiles) {
+  if (!b) return GSR_E_INVALID;
+  if (tiles < 64 || tiles > 4096) return ctx_fail(b->ctx, GSR_E_INVALID, "emit band not in [64, 4096] tiles");
+  if (tiles != b->emit_band) {
+    b->emit_band = tiles;
+    drop_graph(b);  // the band is a launch argument (and the LDS size) of the captured step
+  }
   return GSR_OK;
 }
 
@@ -692,6 +704,22 @@ int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const flo
   }
   int rc = gsr_batch_configure(b, best_tb, best_ch);
   if (rc) return rc;
+  // Then the emit band: small views keep more waves resident with 128 tiles,
+  // large ones rescan their chunks less with 256.
+  const uint32_t emit_bands[] = {128, 256};
+  double best_band_t = 1e30;
+  uint32_t best_band = b->emit_band;
+  for (uint32_t eb : emit_bands) {
+    if ((rc = gsr_batch_set_emit_band(b, eb))) return rc;
+    if ((rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1))) return rc;
+    double dt;
+    if ((rc = time_min(4, dt))) return rc;
+    if (dt < best_band_t) {
+      best_band_t = dt;
+      best_band = eb;
+    }
+  }
+  if ((rc = gsr_batch_set_emit_band(b, best_band))) return rc;
   // Then the walk above which the backward blends a tile as two half-tile items:
   // low enough to shorten the launch's tail, high enough not to repeat each
   // entry's staging for tiles that are merely long (scenes of dense clusters

@@ -89,6 +89,7 @@ struct BinScratch {
   uint2* rect_sorted;    // [V][n] tile rectangles in depth order
   uint64_t* run;         // device scalar: order slots used by the passes so far
   bool packed;           // rectangle format (rect_packed)
+  uint32_t emit_band = 256;  // tiles per band of the emit kernel (a knob: results do not depend on it)
 };
 // One binning pass: the depth slice [pbeg, pend) of every view's sorted
 // Gaussians.  Its entries go to order[ebase[v] + ranges[...]].

@@ -273,7 +273,9 @@ void scan_segments(hipStream_t st, uint32_t* data, size_t seg_len, uint32_t segs
 // before) receive no entries.  The count matrix of a pass is laid out
 // [view][chunk][tile] at offset chunks * tile_base of the view.
 constexpr uint32_t kBandTiles = 2048;      // count kernel: 4 B of LDS per tile
-constexpr uint32_t kEmitBandTiles = 256;   // emit kernel: 12 B of LDS per tile (more resident waves beat fewer rescans)
+// The emit kernel's band (BinScratch::emit_band, 256 tiles unless tuned) costs
+// 12 B of LDS per tile: smaller bands keep more waves resident, larger ones
+// rescan each chunk's rectangles fewer times.
 
 __host__ __device__ inline uint32_t band_rows(uint32_t tiles_x, uint32_t tiles_y, uint32_t budget) {
   uint32_t r = budget / (tiles_x ? tiles_x : 1);
@@ -536,13 +538,13 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
                                                       const uint32_t* __restrict__ matrix,
                                                       const uint64_t* __restrict__ ebase_v,
                                                       uint32_t* __restrict__ order, uint64_t capacity,
-                                                      uint32_t* __restrict__ overflow) {
+                                                      uint32_t* __restrict__ overflow, uint32_t emit_band) {
   extern __shared__ uint4 lds_quads[];
   const uint32_t X = gridDim.x, XY = gridDim.x * gridDim.y;
   const uint32_t logical = emit_block(blockIdx.x + X * blockIdx.y + XY * blockIdx.z, XY * gridDim.z);
   const uint32_t v = logical / XY, c = logical % X, lane = threadIdx.x;
   const ViewDesc vd = views[v];
-  const Band bd = band_of(vd, (logical % XY) / X, kEmitBandTiles);
+  const Band bd = band_of(vd, (logical % XY) / X, emit_band);
   if (!bd.valid) return;
   const uint32_t tx = bd.tx, ntile = (bd.r1 - bd.r0) * tx;
   // queue record: band-local tile of the rectangle's corner, width, pair count
@@ -743,11 +745,11 @@ struct BandGeometry {
   size_t count_lds = 4, emit_lds = 4;
 };
 
-BandGeometry band_geometry(const ViewDesc* host_views, uint32_t num_views) {
+BandGeometry band_geometry(const ViewDesc* host_views, uint32_t num_views, uint32_t emit_band) {
   BandGeometry g;
   for (uint32_t v = 0; v < num_views; ++v) {
     const uint32_t tx = host_views[v].tiles_x, ty = host_views[v].tiles_y;
-    const uint32_t R = band_rows(tx, ty, kBandTiles), Re = band_rows(tx, ty, kEmitBandTiles);
+    const uint32_t R = band_rows(tx, ty, kBandTiles), Re = band_rows(tx, ty, emit_band);
     g.max_bands = std::max(g.max_bands, (ty + R - 1) / R);
     g.max_emit_bands = std::max(g.max_emit_bands, (ty + Re - 1) / Re);
     g.max_tiles = std::max(g.max_tiles, tx * ty);
@@ -766,7 +768,7 @@ void launch_bin_count(hipStream_t s, const ViewDesc* views, const ViewDesc* host
                       uint32_t n, uint32_t chunk, const BinScratch& bs, const BinPass& pass) {
   if (num_views == 0) return;
   const uint32_t chunks = slice_chunks(pass, chunk);
-  const BandGeometry g = band_geometry(host_views, num_views);
+  const BandGeometry g = band_geometry(host_views, num_views, bs.emit_band);
   if (chunks > 0) {
     const dim3 grid(chunks, g.max_bands, num_views);
     if (bs.packed)
@@ -789,18 +791,18 @@ void launch_bin_emit(hipStream_t s, const ViewDesc* views, const ViewDesc* host_
                      uint32_t* order, uint64_t order_capacity, uint32_t* overflow) {
   const uint32_t chunks = slice_chunks(pass, chunk);
   if (num_views == 0 || chunks == 0) return;
-  const BandGeometry g = band_geometry(host_views, num_views);
+  const BandGeometry g = band_geometry(host_views, num_views, bs.emit_band);
   dim3 tgrid((g.max_tiles + 255) / 256, num_views);
   hipLaunchKernelGGL(bin_colscan_kernel, tgrid, dim3(256), 0, s, views, chunks, pass.ranges, bs.matrix);
   const dim3 grid(chunks, g.max_emit_bands, num_views);
   if (bs.packed)
     hipLaunchKernelGGL(bin_emit_kernel<true>, grid, dim3(64), g.emit_lds, s, views, n, chunk, pass.pbeg, pass.pend,
                        chunks, vals, bs.rect_sorted, pass.done, bs.matrix, pass.ebase, order, order_capacity,
-                       overflow);
+                       overflow, bs.emit_band);
   else
     hipLaunchKernelGGL(bin_emit_kernel<false>, grid, dim3(64), g.emit_lds, s, views, n, chunk, pass.pbeg, pass.pend,
                        chunks, vals, bs.rect_sorted, pass.done, bs.matrix, pass.ebase, order, order_capacity,
-                       overflow);
+                       overflow, bs.emit_band);
 }
 
 }  // namespace gsr

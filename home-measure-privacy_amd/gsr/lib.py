@@ -25,7 +25,7 @@ EXPORTS = (
     "gsr_shape", "gsr_adam", "gsr_adam_sh", "gsr_stats", "gsr_batch_create", "gsr_batch_destroy", "gsr_batch_set_cameras",
     "gsr_batch_step", "gsr_batch_entries", "gsr_batch_configure", "gsr_batch_autotune", "gsr_batch_profile",
     "gsr_batch_set_passes", "gsr_loss_grad_window", "gsr_batch_set_ssim_window", "gsr_batch_set_split",
-    "gsr_batch_tuning",
+    "gsr_batch_tuning", "gsr_batch_set_emit_band",
 )
 SSIM_BOX7, SSIM_GAUSS11 = 7, 11
 STAGES = ("project", "depth_sort", "bin_count", "bin_emit", "forward", "loss_grad", "backward", "chain")
@@ -95,6 +95,7 @@ def load():
         "gsr_loss_grad_window": (i32, [P, ctypes.POINTER(Camera), P, P, f32, i32, P]),
         "gsr_batch_set_ssim_window": (i32, [P, i32]),
         "gsr_batch_tuning": (i32, [P, ctypes.POINTER(u32)]),
+        "gsr_batch_set_emit_band": (i32, [P, u32]),
         "gsr_batch_set_split": (i32, [P, u32, u32, u32]),
     }
     for name, (res, args) in sig.items():
@@ -291,15 +292,20 @@ class Batch:
         self.ctx._check(self.lib.gsr_batch_set_split(self.h, forward_first, forward_later, backward))
 
     def tuning(self) -> tuple:
-        """(tile_batch, chunk, split_first, split_later, split_backward) in effect."""
-        out = (ctypes.c_uint32 * 5)()
+        """(tile_batch, chunk, split_first, split_later, split_backward, emit_band) in effect."""
+        out = (ctypes.c_uint32 * 6)()
         self.ctx._check(self.lib.gsr_batch_tuning(self.h, out))
         return tuple(int(x) for x in out)
 
+    def set_emit_band(self, tiles: int):
+        self.ctx._check(self.lib.gsr_batch_set_emit_band(self.h, tiles))
+
     def set_tuning(self, tuning):
-        tb, ch, sf, sl, sb = tuning
+        """Sets what tuning() returned (the emit band may be left off: 256)."""
+        tb, ch, sf, sl, sb = tuning[:5]
         self.configure(tb, ch)
         self.set_split(sf, sl, sb)
+        self.set_emit_band(tuning[5] if len(tuning) > 5 else 256)
 
     def set_ssim_window(self, window: int):
         self.ctx._check(self.lib.gsr_batch_set_ssim_window(self.h, window))

@@ -31,7 +31,7 @@ def small_batch_scene(n=20000, views=5, seed=0, sizes=None):
 
 
 def run_batch(ctx, sc, graph=True, tile_batch=0, chunk=0, steps=1, images=False, passes=None, growth=2, split=None,
-              lam=0.0):
+              lam=0.0, emit_band=None):
     n = sc.n
     gsb, shb = ctx.upload(sc.gs), ctx.upload(sc.sh)
     tgt = ctx.upload(np.concatenate([t.ravel() for t in sc.targets]).astype(np.float32))
@@ -44,6 +44,8 @@ def run_batch(ctx, sc, graph=True, tile_batch=0, chunk=0, steps=1, images=False,
         b.set_passes(passes, growth)
     if split is not None:
         b.set_split(*split)
+    if emit_band is not None:
+        b.set_emit_band(emit_band)
     outs = []
     for _ in range(steps):
         b.step(gsb, shb, tgt, lam, grads, shape, shg, img, graph)
@@ -149,11 +151,15 @@ def test_batch_tuning_reads_back_the_knobs(ctx):
     """gsr_batch_tuning returns what configure / set_split set (0 keeps a knob)."""
     sc = small_batch_scene(n=3000, views=2, seed=4)
     b = lib.Batch(ctx, sc.cams, sc.n, 3)
-    b.set_tuning((4, 2048, 100, 200, 300))
-    assert b.tuning() == (4, 2048, 100, 200, 300)
+    b.set_tuning((4, 2048, 100, 200, 300, 128))
+    assert b.tuning() == (4, 2048, 100, 200, 300, 128)
     b.configure(0, 0)
     b.set_split(0, 0, 0)
-    assert b.tuning() == (4, 2048, 0, 0, 0)
+    assert b.tuning() == (4, 2048, 0, 0, 0, 128)
+    b.set_tuning((1, 1024, 0, 0, 1024))  # the emit band left off: 256
+    assert b.tuning() == (1, 1024, 0, 0, 1024, 256)
+    with pytest.raises(lib.GsrError):
+        b.set_emit_band(32)
     b.close()
 
 
@@ -164,9 +170,11 @@ def test_batch_graph_replay_and_knobs(ctx):
     eager, e0 = run_batch(ctx, sc, graph=False)
     replay, e1 = run_batch(ctx, sc, graph=True, steps=3)
     tuned, e2 = run_batch(ctx, sc, graph=True, tile_batch=4, chunk=512, steps=2)
+    banded, e3 = run_batch(ctx, sc, graph=True, emit_band=64, steps=2)  # 64-tile emit bands
     np.testing.assert_array_equal(e0, e1)
     np.testing.assert_array_equal(e0, e2)
-    for outs in (replay, tuned):
+    np.testing.assert_array_equal(e0, e3)
+    for outs in (replay, tuned, banded):
         for g, s, sh, _ in outs:
             assert rel_l2(g[:, :4], eager[0][0][:, :4]) < 1e-5
             assert rel_l2(s, eager[0][1]) < 1e-5
