@@ -105,6 +105,75 @@ def plain_path(ctx, sc, gsb, shb, tgt, n, passes):
     return passes * len(sc.cams) / dt
 
 
+def batched_per_view(ctx, sc, gsb, shb, tgt, n, tuning, passes):
+    """The batched machinery at one view per batch (hit-box binning, depth slices,
+    persistent blends, graph replay), the views run one after another with the
+    32-view batch's knobs: separates the gain of batching views from the binning
+    savings the plain path lacks."""
+    offs = np.concatenate([[0], np.cumsum([c.pixels * 12 for c in sc.cams])])
+    grads, shape, shg = ctx.alloc(n * 64), ctx.alloc(n * 40), ctx.alloc(n * 192)
+    batches = []
+    for v, cam in enumerate(sc.cams):
+        b = lib.Batch(ctx, [cam], n, 3)
+        b.set_tuning(tuning)
+        if passes:
+            b.set_passes(passes[0], passes[1])
+        batches.append((b, sub(tgt, int(offs[v]), cam.pixels * 12)))
+
+    def one_pass():
+        for b, t in batches:
+            b.step(gsb, shb, t, 0.0, grads, shape, shg, None, True)
+
+    one_pass()  # sizes the entry storage and captures the graphs
+    one_pass()
+    ctx.sync()
+    t0 = time.perf_counter()
+    one_pass()
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    for b, _ in batches:
+        b.close()
+    for x in (grads, shape, shg):
+        x.free()
+    return len(sc.cams) / dt
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_cfg0(threads: int, budget_s: float = 3.0):
+    """The reference's own CPU-runnable case, gradients() (gpu.js:826-839) at 10k
+    SH0 Gaussians, 256x256: project, tileLists, forward, L1 gradient, backward
+    with atomic adds, SHAPE by central differences, on `threads` host threads.
+    Fastest of repeated runs within budget_s, as views/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    sc = scenes.config(0)
+    cam, target = sc.cams[0], sc.targets[0]
+    gs = sc.gs.astype(np.float32)
+    used = O.threads(threads)
+    best, t_all = 1e30, time.perf_counter()
+    while time.perf_counter() - t_all < budget_s:
+        t0 = time.perf_counter()
+        splats = O.project(cam, gs, None, 0, np.float32)
+        lst, ranges = O.tile_lists(splats, cam.width, cam.height)
+        img, st = O.forward(splats, lst, ranges, cam.width, cam.height, np.float32)
+        dl = O.loss_grad(img, target, 0.0, np.float32)
+        grads = O.backward(splats, lst, ranges, st, dl, cam.width, cam.height, np.float32, threaded=True)
+        O.shape_fd(cam, gs, grads, np.float32)
+        best = min(best, time.perf_counter() - t0)
+    return used, 1.0 / best
+
+
 def cpu_baseline(sc, budget_s: float):
     """FP32 restatement of the reference's one-view step on the host cores
     (project, tileLists, forward, L1 gradient, backward with atomic adds, SHAPE by
@@ -113,6 +182,12 @@ def cpu_baseline(sc, budget_s: float):
     import oracle as O
 
     cores = O.threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    # The reference's CPU-runnable case first: all threads, then one.
+    _, cfg0_all = cpu_cfg0(cores)
+    _, cfg0_one = cpu_cfg0(1)
+    O.threads(cores)
+    log(f"  cpu cfg 0 (10k SH0, 256x256, one fwd+bwd): {cfg0_all:.2f} views/s on {cores} threads, "
+        f"{cfg0_one:.2f} on 1")
     done, t0 = 0, time.perf_counter()
     for cam, target in zip(sc.cams, sc.targets):
         splats = O.project(cam, sc.gs.astype(np.float32), sc.sh.astype(np.float32), 3, np.float32)
@@ -130,7 +205,12 @@ def cpu_baseline(sc, budget_s: float):
     return {"value": done / dt, "unit": "views/s", "cores": cores, "kind": "port",
             "sample": f"first {done} of the {len(sc.cams)} views ({', '.join(sizes)}), all {sc.n} Gaussians, "
                       f"FP32 restatement of the reference step (SH3 colour, SHAPE by central differences), "
-                      f"{dt:.1f} s"}
+                      f"{dt:.1f} s",
+            "cpu_model": cpu_model(), "host_cpus": len(os.sched_getaffinity(0)),
+            "cfg0_views_per_s": {"threads": cores, "value": round(cfg0_all, 3)},
+            "cfg0_views_per_s_1_thread": round(cfg0_one, 3),
+            "cfg0_sample": "10k SH0 Gaussians, 256x256, one forward + backward (gradients(), gpu.js:826-839), "
+                           "FP32 restatement, fastest of repeats over 3 s"}
 
 
 def copy_calibration(mib: int):
@@ -310,17 +390,27 @@ def main():
     achieved = bytes_[dominant] / (dom_ms * 1e-3) / 1e9
     bwd_achieved = bytes_["backward"] / (timed["backward"] * 1e-3) / 1e9
     # HBM bytes per step of the dominant stage's kernel(s), from the rocprofv3
-    # FETCH_SIZE / WRITE_SIZE passes committed for this workload (copy-calibrated).
-    traffic = None
+    # FETCH_SIZE / WRITE_SIZE passes committed for this workload (copy-calibrated),
+    # profiled per tuning: reported only for the knobs this run timed.
+    traffic, traffic_note = None, None
     tpath = os.path.join(ROOT, "profiles", "traffic_batched_1M_32views.json")
+    key = ",".join(str(x) for x in tuning)
     if (n, V) == (1_000_000, 32) and os.path.exists(tpath):
-        traffic = json.load(open(tpath))["stages"].get(dominant)
-        traffic = None if traffic is None else round(traffic)
+        by_tuning = json.load(open(tpath)).get("by_tuning", {})
+        if key in by_tuning:
+            t = by_tuning[key]["stages"].get(dominant)
+            traffic = None if t is None else round(t)
+        else:
+            traffic_note = f"no FETCH_SIZE / WRITE_SIZE profile for tuning {key} (profiled: {sorted(by_tuning)})"
 
-    plain = None
+    plain = batched_v1 = None
     if not args.no_plain:
         plain = plain_path(ctx, sc, gsb, shb, tgt, n, args.plain_passes)
         log(f"plain: {plain:.1f} views/s  (batched / plain = {value / plain:.2f}x)")
+        passes = (args.passes, args.growth) if args.passes else None
+        batched_v1 = batched_per_view(ctx, sc, gsb, shb, tgt, n, tuning, passes)
+        log(f"batched machinery one view at a time: {batched_v1:.1f} views/s  (32-view batch / that = "
+            f"{value / batched_v1:.2f}x; that / plain = {batched_v1 / plain:.2f}x)")
     batch.close()
     for b in (gsb, shb, tgt, grads, shape, shg):
         b.free()
@@ -344,6 +434,9 @@ def main():
             "walked_entries_per_step": int(Wk),
             "plain_views_per_s": None if plain is None else round(plain, 3),
             "batched_over_plain": None if plain is None else round(value / plain, 3),
+            "batched_v1_views_per_s": None if batched_v1 is None else round(batched_v1, 3),
+            "gain_from_batching_views": None if batched_v1 is None else round(value / batched_v1, 3),
+            "gain_from_binning_and_blend": None if plain is None else round(batched_v1 / plain, 3),
             "gaussians_projected_per_s": round(n * V / (timed["project"] * 1e-3), 1),
             "stage_ms": {k: round(v, 4) for k, v in timed.items()},
             "stage_GBps": {k: round(v, 1) for k, v in stage_bw.items()},
@@ -352,7 +445,8 @@ def main():
             "depth_passes": args.passes or "default",
         },
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     **({"traffic_note": traffic_note} if traffic_note else {})},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

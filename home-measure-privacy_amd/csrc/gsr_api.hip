@@ -25,6 +25,8 @@ struct gsr_ctx {
   uint32_t work_w = 0, work_h = 0;
   // Pinned staging for small device->host reads.
   uint64_t* pinned = nullptr;
+  // Verification knob of gsr_forward / gsr_backward: no hit-box culling.
+  int no_cull = 0;
 };
 
 namespace gsr {
@@ -238,6 +240,12 @@ int gsr_copy(gsr_ctx* ctx, void* dst, const void* src, size_t bytes) {
   return GSR_OK;
 }
 
+int gsr_set_culling(gsr_ctx* ctx, int enabled) {
+  if (!ctx) return GSR_E_INVALID;
+  ctx->no_cull = enabled ? 0 : 1;
+  return GSR_OK;
+}
+
 int gsr_synchronize(gsr_ctx* ctx) {
   if (!ctx) return GSR_E_INVALID;
   GSR_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -260,53 +268,92 @@ int gsr_project(gsr_ctx* ctx, const gsr_camera* cam, const float* gs, const floa
   return GSR_OK;
 }
 
-int gsr_bin_sort(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, uint32_t n, uint32_t* order,
-                 uint64_t order_capacity, uint32_t* ranges, uint64_t* num_entries) {
-  if (!ctx) return GSR_E_INVALID;
-  if (!ranges || !num_entries || (n && !splats)) return fail(ctx, GSR_E_INVALID, "NULL argument");
-  if (order_capacity && !order) return fail(ctx, GSR_E_INVALID, "order is NULL with capacity %llu",
-                                            (unsigned long long)order_capacity);
-  ViewDesc* dview;
-  gsr_camera* dcam;
-  ViewDesc host;
-  int rc = single_view(ctx, cam, &dview, &dcam, nullptr, &host);
-  if (rc) return rc;
-  const size_t nn = std::max<uint32_t>(n, 1);
+}  // extern "C"
+
+namespace {
+
+// Device scratch of one gsr_bin_sort call.
+struct BinSortScratch {
   uint32_t *keys, *keys_alt, *vals, *vals_alt, *hist, *piece, *matrix, *tile_tot, *overflow;
   uint2 *rects, *rect_sorted;
   uint64_t *ebase, *entries, *run;
-  const uint32_t chunk = pick_chunk(n);
-  const size_t hist_n = std::max<size_t>(1, radix_hist_elems(n, 1));
-  const size_t piece_n = std::max<size_t>(1, scan_piece_elems(hist_n, 1));
-  const size_t mat_n = std::max<size_t>(1, bin_matrix_elems(&host, 1, n, chunk));
-  if ((rc = pooled_t(ctx, "keys", nn, &keys)) || (rc = pooled_t(ctx, "keys_alt", nn, &keys_alt)) ||
-      (rc = pooled_t(ctx, "vals", nn, &vals)) || (rc = pooled_t(ctx, "vals_alt", nn, &vals_alt)) ||
-      (rc = pooled_t(ctx, "rects", nn, &rects)) || (rc = pooled_t(ctx, "rect_sorted", nn, &rect_sorted)) ||
-      (rc = pooled_t(ctx, "hist", hist_n, &hist)) ||
-      (rc = pooled_t(ctx, "piece", piece_n, &piece)) || (rc = pooled_t(ctx, "matrix", mat_n, &matrix)) ||
-      (rc = pooled_t(ctx, "tile_tot", host.num_tiles, &tile_tot)) || (rc = pooled_t(ctx, "entries", 1, &entries)) ||
-      (rc = pooled_t(ctx, "ebase", 1, &ebase)) || (rc = pooled_t(ctx, "run", 1, &run)) ||
-      (rc = pooled_t(ctx, "overflow", 1, &overflow)))
-    return rc;
-  launch_zero(ctx->stream, overflow, sizeof(uint32_t));
+};
+
+// Element counts of each scratch region for n Gaussians binned on `host`.
+struct BinSortSizes {
+  size_t nn, hist, piece, matrix, tiles;
+  uint32_t chunk;
+};
+
+BinSortSizes bin_sort_sizes(const ViewDesc& host, uint32_t n) {
+  BinSortSizes z;
+  z.nn = std::max<uint32_t>(n, 1);
+  z.chunk = pick_chunk(n);
+  z.hist = std::max<size_t>(1, radix_hist_elems(n, 1));
+  z.piece = std::max<size_t>(1, scan_piece_elems(z.hist, 1));
+  z.matrix = std::max<size_t>(1, bin_matrix_elems(&host, 1, n, z.chunk));
+  z.tiles = std::max<uint32_t>(host.num_tiles, 1);
+  return z;
+}
+
+// tileLists (gpu.js:450-481) over the scratch `w`: prep, depth sort, counting
+// binning, the entry count back to the host, then the order when it fits.
+int bin_sort_run(gsr_ctx* ctx, const ViewDesc& host, const ViewDesc* dview, const float* splats, uint32_t n,
+                 uint32_t* order, uint64_t order_capacity, uint32_t* ranges, uint64_t* num_entries,
+                 const BinSortScratch& w, uint32_t chunk) {
+  launch_zero(ctx->stream, w.overflow, sizeof(uint32_t));
   const bool packed = rect_packed(std::max(host.tiles_x, host.tiles_y));
-  launch_prep(ctx->stream, dview, 1, splats, n, keys, rects, packed);
-  SortScratch sc{keys_alt, vals, vals_alt, hist, piece};
-  launch_depth_sort(ctx->stream, keys, n, 1, sc, rects, rect_sorted, packed);
-  BinScratch bs{matrix, tile_tot, rect_sorted, run, packed};
-  BinPass pass{0, n, nullptr, ranges, ebase, entries, true};
+  launch_prep(ctx->stream, dview, 1, splats, n, w.keys, w.rects, packed);
+  SortScratch sc{w.keys_alt, w.vals, w.vals_alt, w.hist, w.piece};
+  launch_depth_sort(ctx->stream, w.keys, n, 1, sc, w.rects, w.rect_sorted, packed);
+  BinScratch bs{w.matrix, w.tile_tot, w.rect_sorted, w.run, packed};
+  BinPass pass{0, n, nullptr, ranges, w.ebase, w.entries, true};
   launch_bin_count(ctx->stream, dview, &host, 1, n, chunk, bs, pass);
   GSR_LAUNCHED(ctx);
-  GSR_HIP(ctx, hipMemcpyAsync(ctx->pinned, entries, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  GSR_HIP(ctx, hipMemcpyAsync(ctx->pinned, w.entries, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
   GSR_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const uint64_t E = ctx->pinned[0];
   *num_entries = E;
   if (E > order_capacity)
     return fail(ctx, GSR_E_CAPACITY, "order holds %llu entries, %llu needed", (unsigned long long)order_capacity,
                 (unsigned long long)E);
-  launch_bin_emit(ctx->stream, dview, &host, 1, n, chunk, vals, bs, pass, order, order_capacity, overflow);
+  launch_bin_emit(ctx->stream, dview, &host, 1, n, chunk, w.vals, bs, pass, order, order_capacity, w.overflow);
   GSR_LAUNCHED(ctx);
   return GSR_OK;
+}
+
+int check_bin_sort_args(gsr_ctx* ctx, const float* splats, uint32_t n, uint32_t* order, uint64_t order_capacity,
+                        uint32_t* ranges, uint64_t* num_entries) {
+  if (!ranges || !num_entries || (n && !splats)) return fail(ctx, GSR_E_INVALID, "NULL argument");
+  if (order_capacity && !order)
+    return fail(ctx, GSR_E_INVALID, "order is NULL with capacity %llu", (unsigned long long)order_capacity);
+  return GSR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsr_bin_sort(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, uint32_t n, uint32_t* order,
+                 uint64_t order_capacity, uint32_t* ranges, uint64_t* num_entries) {
+  if (!ctx) return GSR_E_INVALID;
+  int rc = check_bin_sort_args(ctx, splats, n, order, order_capacity, ranges, num_entries);
+  if (rc) return rc;
+  ViewDesc* dview;
+  gsr_camera* dcam;
+  ViewDesc host;
+  if ((rc = single_view(ctx, cam, &dview, &dcam, nullptr, &host))) return rc;
+  const BinSortSizes z = bin_sort_sizes(host, n);
+  BinSortScratch w;
+  if ((rc = pooled_t(ctx, "keys", z.nn, &w.keys)) || (rc = pooled_t(ctx, "keys_alt", z.nn, &w.keys_alt)) ||
+      (rc = pooled_t(ctx, "vals", z.nn, &w.vals)) || (rc = pooled_t(ctx, "vals_alt", z.nn, &w.vals_alt)) ||
+      (rc = pooled_t(ctx, "rects", z.nn, &w.rects)) || (rc = pooled_t(ctx, "rect_sorted", z.nn, &w.rect_sorted)) ||
+      (rc = pooled_t(ctx, "hist", z.hist, &w.hist)) || (rc = pooled_t(ctx, "piece", z.piece, &w.piece)) ||
+      (rc = pooled_t(ctx, "matrix", z.matrix, &w.matrix)) || (rc = pooled_t(ctx, "tile_tot", z.tiles, &w.tile_tot)) ||
+      (rc = pooled_t(ctx, "entries", 1, &w.entries)) || (rc = pooled_t(ctx, "ebase", 1, &w.ebase)) ||
+      (rc = pooled_t(ctx, "run", 1, &w.run)) || (rc = pooled_t(ctx, "overflow", 1, &w.overflow)))
+    return rc;
+  return bin_sort_run(ctx, host, dview, splats, n, order, order_capacity, ranges, num_entries, w, z.chunk);
 }
 
 static int raster_common(gsr_ctx* ctx, const gsr_camera* cam, RasterArgs& a, ViewDesc& host) {
@@ -327,6 +374,7 @@ static int raster_common(gsr_ctx* ctx, const gsr_camera* cam, RasterArgs& a, Vie
   a.pass = 0;
   a.tstate = nullptr;
   a.done = nullptr;
+  a.no_cull = ctx->no_cull;
   return GSR_OK;
 }
 

@@ -79,6 +79,8 @@ struct gsr_batch {
   // tiles walk their whole segment) and the backward (entries the forward walked).
   uint32_t split_first = 0, split_later = 0, split_backward = 1024;
   uint32_t emit_band = 256;  // tiles per band of the emit kernel
+  int no_cull = 0;           // verification knob: no hit-box binning or culling
+  uint64_t* valid_mask = nullptr;  // [V][ceil(n / 64)]: projected validly (STATS)
   uint32_t* order = nullptr;
   uint64_t order_cap = 0;
   // [0] overflow, [16 + p] split-item count of forward pass p, [16 + kMaxPasses]
@@ -172,7 +174,8 @@ int alloc_all(gsr_batch* b) {
       (rc = dalloc(b, &b->heavy, size_t(b->total_tiles) * 2)) || (rc = dalloc(b, &b->split, b->total_tiles)) ||
       (rc = dalloc(b, &b->flags, kFlagWords)) || (rc = dalloc(b, &b->image, b->total_pixels * 3)) ||
       (rc = dalloc(b, &b->dl, b->total_pixels * 3)) ||
-      (rc = dalloc(b, &b->vgrads, size_t(b->V) * std::max<uint32_t>(b->n, 1) * kRecord)))
+      (rc = dalloc(b, &b->vgrads, size_t(b->V) * std::max<uint32_t>(b->n, 1) * kRecord)) ||
+      (rc = dalloc(b, &b->valid_mask, size_t(b->V) * valid_mask_words(std::max<uint32_t>(b->n, 1)))))
     return rc;
   (void)s;
   return GSR_OK;
@@ -221,7 +224,7 @@ struct Prof {
 
 // Issues one full step on the stream (capturable: kernels only).
 void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* targets, float lambda, float* grads,
-                float* shape, float* sh_grad, float* images, Prof* prof = nullptr) {
+                float* shape, float* sh_grad, float* stats, float* images, Prof* prof = nullptr) {
   hipStream_t s = ctx_stream(b->ctx);
   float* image = images ? images : b->image;
   int stage = 0;
@@ -239,7 +242,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   for (const ViewDesc& vd : b->views) axis = std::max(axis, std::max(vd.tiles_x, vd.tiles_y));
   const bool packed = rect_packed(axis);
   launch_project(s, b->d_cams, b->d_views, b->V, gs, sh, b->deg, b->n, b->splats, b->keys, b->rects, nullptr,
-                 packed);
+                 packed, !b->no_cull, b->valid_mask);
   mark();
   SortScratch sc{b->keys_alt, b->vals, b->vals_alt, b->hist, b->piece};
   launch_depth_sort(s, b->keys, b->n, b->V, sc, b->rects, b->rect_sorted, packed);
@@ -266,6 +269,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   a.tstate = b->num_passes > 1 ? b->tstate : nullptr;
   a.done = b->num_passes > 1 ? b->done : nullptr;
   a.band_open = b->num_passes > 1 ? b->band_open : nullptr;
+  a.no_cull = b->no_cull;
   // Binning and forward alternate per depth slice: a slice is binned only for
   // the tiles the slices before it left unsaturated.  Profiled steps time all
   // binning as "bin count" + "bin emit" and all forward passes as "forward".
@@ -357,12 +361,12 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   launch_backward(s, a, 1, b->grid_bwd);
   mark();
   launch_shape(s, b->d_cams, b->V, gs, sh, b->deg, b->n, b->vgrads, uint64_t(b->n) * kRecord, shape, sh_grad, grads,
-               1, prof && prof->last ? prof->touched : nullptr, kRecord);
+               1, prof && prof->last ? prof->touched : nullptr, kRecord, b->valid_mask, stats, b->flags);
   mark();
 }
 
 int run_step(gsr_batch* b, const float* gs, const float* sh, const float* targets, float lambda, float* grads,
-             float* shape, float* sh_grad, float* images, int use_graph) {
+             float* shape, float* sh_grad, float* stats, float* images, int use_graph) {
   hipStream_t s = ctx_stream(b->ctx);
   if (lambda > 0.0f && !b->coef) {
     int rc = dalloc(b, &b->coef, b->total_pixels * 9);
@@ -371,12 +375,12 @@ int run_step(gsr_batch* b, const float* gs, const float* sh, const float* target
   }
   for (int attempt = 0; attempt < 3; ++attempt) {
     if (use_graph) {
-      std::vector<const void*> key = {gs, sh, targets, grads, shape, sh_grad, images};
+      std::vector<const void*> key = {gs, sh, targets, grads, shape, sh_grad, stats, images};
       if (!b->exec || key != b->graph_key || lambda != b->graph_lambda) {
         drop_graph(b);
         if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
           return ctx_fail(b->ctx, GSR_E_HIP, "hipStreamBeginCapture failed");
-        issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, images);
+        issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, stats, images);
         hipError_t e = hipStreamEndCapture(s, &b->graph);
         if (e != hipSuccess) return ctx_fail(b->ctx, GSR_E_HIP, "hipStreamEndCapture failed");
         if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
@@ -386,7 +390,7 @@ int run_step(gsr_batch* b, const float* gs, const float* sh, const float* target
       }
       if (hipGraphLaunch(b->exec, s) != hipSuccess) return ctx_fail(b->ctx, GSR_E_HIP, "hipGraphLaunch failed");
     } else {
-      issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, images);
+      issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, stats, images);
       if (hipGetLastError() != hipSuccess) return ctx_fail(b->ctx, GSR_E_HIP, "kernel launch failed");
     }
     const size_t ne = size_t(b->num_passes) * b->V;
@@ -487,13 +491,56 @@ int gsr_batch_set_cameras(gsr_batch* b, const gsr_camera* cams) {
   return upload_cams(b);
 }
 
-int gsr_batch_step(gsr_batch* b, const float* gs, const float* sh, const float* targets, float lambda, float* grads,
-                   float* shape, float* sh_grad, float* images, int use_graph) {
+int gsr_batch_step_stats(gsr_batch* b, const float* gs, const float* sh, const float* targets, float lambda,
+                         float* grads, float* shape, float* sh_grad, float* stats, float* images, int use_graph) {
   if (!b) return GSR_E_INVALID;
   if ((b->n && (!gs || !grads || !shape)) || !targets) return ctx_fail(b->ctx, GSR_E_INVALID, "NULL argument");
   if (b->deg > 0 && (!sh || !sh_grad)) return ctx_fail(b->ctx, GSR_E_INVALID, "sh / sh_grad is NULL");
   if (!(lambda >= 0.0f && lambda <= 1.0f)) return ctx_fail(b->ctx, GSR_E_INVALID, "lambda not in [0, 1]");
-  return run_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, images, use_graph);
+  return run_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, stats, images, use_graph);
+}
+
+int gsr_batch_step(gsr_batch* b, const float* gs, const float* sh, const float* targets, float lambda, float* grads,
+                   float* shape, float* sh_grad, float* images, int use_graph) {
+  return gsr_batch_step_stats(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, images, use_graph);
+}
+
+int gsr_batch_set_culling(gsr_batch* b, int enabled) {
+  if (!b) return GSR_E_INVALID;
+  const int no_cull = enabled ? 0 : 1;
+  if (no_cull != b->no_cull) {
+    b->no_cull = no_cull;
+    drop_graph(b);  // the flag is a launch argument of the captured step
+  }
+  return GSR_OK;
+}
+
+int gsr_batch_tile_lists(gsr_batch* b, uint32_t view, uint32_t pass, uint32_t* order, uint64_t order_capacity,
+                         uint32_t* ranges, uint64_t* num_entries) {
+  if (!b) return GSR_E_INVALID;
+  if (!ranges || !num_entries) return ctx_fail(b->ctx, GSR_E_INVALID, "NULL argument");
+  if (view >= b->V || pass >= b->num_passes) return ctx_fail(b->ctx, GSR_E_INVALID, "view or pass out of range");
+  if (order_capacity && !order) return ctx_fail(b->ctx, GSR_E_INVALID, "order is NULL with a nonzero capacity");
+  hipStream_t s = ctx_stream(b->ctx);
+  const size_t k = size_t(pass) * b->V + view;
+  uint64_t ebase = 0, count = 0;
+  if (hipMemcpyAsync(&ebase, b->ebase + k, sizeof(uint64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&count, b->entries + k, sizeof(uint64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return ctx_fail(b->ctx, GSR_E_HIP, "reading the entry counts failed");
+  *num_entries = count;
+  const ViewDesc& vd = b->views[view];
+  const size_t rbytes = size_t(vd.num_tiles) * 2 * sizeof(uint32_t);
+  if (hipMemcpyAsync(ranges, b->ranges + (size_t(pass) * b->total_tiles + vd.tile_base) * 2, rbytes,
+                     hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return ctx_fail(b->ctx, GSR_E_HIP, "copying the ranges failed");
+  if (count > order_capacity) return ctx_fail(b->ctx, GSR_E_CAPACITY, "order is too small for the view's entries");
+  if (ebase + count > b->order_cap) return ctx_fail(b->ctx, GSR_E_INVALID, "no complete step to read");
+  if (count && hipMemcpyAsync(order, b->order + ebase, count * sizeof(uint32_t), hipMemcpyDeviceToDevice, s) !=
+                   hipSuccess)
+    return ctx_fail(b->ctx, GSR_E_HIP, "copying the order failed");
+  if (hipStreamSynchronize(s) != hipSuccess) return ctx_fail(b->ctx, GSR_E_HIP, "copying the tile lists failed");
+  return GSR_OK;
 }
 
 int gsr_batch_entries(gsr_batch* b, uint64_t* entries) {
@@ -569,7 +616,7 @@ int gsr_batch_profile(gsr_batch* b, const float* gs, const float* sh, const floa
   uint32_t* last = prof.last;
   prof.last = nullptr;
   if (e == hipSuccess) {
-    issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, &prof);
+    issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, nullptr, &prof);
     keep(hipStreamSynchronize(s));
   }
   float binned[3] = {prof.binned_ms[0], prof.binned_ms[1], prof.binned_ms[2]};
@@ -582,7 +629,7 @@ int gsr_batch_profile(gsr_batch* b, const float* gs, const float* sh, const floa
     keep(hipEventElapsedTime(marks + 4, prof.ev[5], prof.ev[6]));
     prof.last = last;
     launch_zero(s, prof.touched, b->V * 4);
-    issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, &prof);
+    issue_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, nullptr, &prof);
     keep(hipStreamSynchronize(s));
   }
   if (e == hipSuccess) {
@@ -670,120 +717,86 @@ int gsr_batch_set_emit_band(gsr_batch* b, uint32_t tiles) {
 int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const float* targets, float lambda,
                        float* grads, float* shape, float* sh_grad, uint32_t* tile_batch, uint32_t* chunk) {
   if (!b) return GSR_E_INVALID;
-  // The shortest of `reps` single steps (each returns after its stream is done):
+  // The shortest of kReps single steps (each returns after its stream is done):
   // the minimum is steadier than a mean against the host's own hiccups.
-  auto time_min = [&](int reps, double& dt) {
+  constexpr int kReps = 6;
+  // A candidate replaces the setting in effect only when it is more than 1%
+  // faster, so near-ties keep the current knobs and the choice is repeatable.
+  constexpr double kMargin = 0.99;
+  auto time_min = [&](double& dt) {
     dt = 1e30;
-    for (int k = 0; k < reps; ++k) {
+    // Warm: captures the graph for the knobs in effect.
+    int rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
+    if (rc) return rc;
+    for (int k = 0; k < kReps; ++k) {
       const auto t0 = std::chrono::steady_clock::now();
-      const int rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
+      rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
       if (rc) return rc;
       dt = std::min(dt, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
     }
     return int(GSR_OK);
   };
-  const uint32_t tbs[] = {1, 2, 4, 8};
-  const uint32_t chunks[] = {1024, 2048, 4096, 8192};
-  double best = 1e30;
-  uint32_t best_tb = b->tile_batch, best_ch = b->chunk;
-  for (uint32_t ch : chunks) {
-    for (uint32_t tb : tbs) {
-      int rc = gsr_batch_configure(b, tb, ch);
-      if (rc) return rc;
-      // Warm (captures the graph), then time replays.
-      rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
+  // One knob at a time: `apply` sets candidate k, the first candidate being the
+  // setting in effect.
+  auto pick = [&](size_t count, auto&& apply, size_t& best_k) {
+    double best = 1e30;
+    best_k = 0;
+    for (size_t k = 0; k < count; ++k) {
+      int rc = apply(k);
       if (rc) return rc;
       double dt;
-      if ((rc = time_min(4, dt))) return rc;
-      if (dt < best) {
+      if ((rc = time_min(dt))) return rc;
+      if (k == 0 || dt < best * kMargin) {
         best = dt;
-        best_tb = tb;
-        best_ch = ch;
+        best_k = k;
       }
     }
-  }
-  int rc = gsr_batch_configure(b, best_tb, best_ch);
-  if (rc) return rc;
-  // Then the emit band: small views keep more waves resident with 128 tiles,
-  // large ones rescan their chunks less with 256.
-  const uint32_t emit_bands[] = {128, 256};
-  double best_band_t = 1e30;
-  uint32_t best_band = b->emit_band;
-  for (uint32_t eb : emit_bands) {
-    if ((rc = gsr_batch_set_emit_band(b, eb))) return rc;
-    if ((rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1))) return rc;
-    double dt;
-    if ((rc = time_min(4, dt))) return rc;
-    if (dt < best_band_t) {
-      best_band_t = dt;
-      best_band = eb;
-    }
-  }
-  if ((rc = gsr_batch_set_emit_band(b, best_band))) return rc;
-  // Then the walk above which the backward blends a tile as two half-tile items:
-  // low enough to shorten the launch's tail, high enough not to repeat each
-  // entry's staging for tiles that are merely long (scenes of dense clusters
-  // prefer a higher one).
-  const uint32_t splits[] = {512, 1024, 2048, 4096};
-  double best_split_t = 1e30;
-  uint32_t best_split = b->split_backward;
-  for (uint32_t sp : splits) {
-    b->split_backward = sp;
-    drop_graph(b);  // the threshold is a launch argument of the captured step
-    rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
-    if (rc) return rc;
-    double dt;
-    if ((rc = time_min(4, dt))) return rc;
-    if (dt < best_split_t) {
-      best_split_t = dt;
-      best_split = sp;
-    }
-  }
-  b->split_backward = best_split;
-  drop_graph(b);
-  // And whether (and from which list length) the forward's later slices split
-  // their tiles: their unsaturated tiles walk whole segments, so a few long
-  // lists can end the launch late; splitting them pays on some scenes only.
-  const uint32_t later[] = {0, 1024, 2048, 4096};
-  double best_later_t = 1e30;
-  uint32_t best_later = b->split_later;
-  for (uint32_t sl : later) {
-    b->split_later = sl;
-    drop_graph(b);
-    rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
-    if (rc) return rc;
-    double dt;
-    if ((rc = time_min(4, dt))) return rc;
-    if (dt < best_later_t) {
-      best_later_t = dt;
-      best_later = sl;
-    }
-  }
-  b->split_later = best_later;
-  drop_graph(b);
+    return apply(best_k);
+  };
+  int rc;
+  size_t k;
+  // Tile batch x binning chunk.
+  std::vector<std::pair<uint32_t, uint32_t>> grid = {{b->tile_batch, b->chunk}};
+  for (uint32_t ch : {1024u, 2048u, 4096u, 8192u})
+    for (uint32_t tb : {1u, 2u, 4u, 8u})
+      if (tb != b->tile_batch || ch != b->chunk) grid.push_back({tb, ch});
+  if ((rc = pick(grid.size(), [&](size_t i) { return gsr_batch_configure(b, grid[i].first, grid[i].second); }, k)))
+    return rc;
+  // The emit band: small views keep more waves resident with 128 tiles, large
+  // ones rescan their chunks less with 256.
+  std::vector<uint32_t> bands = {b->emit_band};
+  for (uint32_t eb : {128u, 256u})
+    if (eb != b->emit_band) bands.push_back(eb);
+  if ((rc = pick(bands.size(), [&](size_t i) { return gsr_batch_set_emit_band(b, bands[i]); }, k))) return rc;
+  // The walk above which the backward blends a tile as two half-tile items: low
+  // enough to shorten the launch's tail, high enough not to repeat each entry's
+  // staging for tiles that are merely long (scenes of dense clusters prefer a
+  // higher one).
+  auto values = [](uint32_t current, std::initializer_list<uint32_t> cands) {
+    std::vector<uint32_t> v = {current};
+    for (uint32_t c : cands)
+      if (c != current) v.push_back(c);
+    return v;
+  };
+  const std::vector<uint32_t> backs = values(b->split_backward, {512u, 1024u, 2048u, 4096u});
+  if ((rc = pick(backs.size(),
+                 [&](size_t i) { return gsr_batch_set_split(b, b->split_first, b->split_later, backs[i]); }, k)))
+    return rc;
+  // Whether (and from which list length) the forward's later slices split their
+  // tiles: their unsaturated tiles walk whole segments, so a few long lists can
+  // end the launch late; splitting them pays on some scenes only.
+  const std::vector<uint32_t> laters = values(b->split_later, {0u, 1024u, 2048u, 4096u});
+  if ((rc = pick(laters.size(),
+                 [&](size_t i) { return gsr_batch_set_split(b, b->split_first, laters[i], b->split_backward); }, k)))
+    return rc;
   // The first slice by the previous step's walk (its tiles mostly saturate early,
-  // so their list lengths say little).
-  const uint32_t firsts[] = {0, 1024, 2048};
-  double best_first_t = 1e30;
-  uint32_t best_first = b->split_first;
-  for (uint32_t sf : firsts) {
-    b->split_first = sf;
-    drop_graph(b);
-    for (int k = 0; k < 2; ++k) {  // the second fills the walk estimate
-      rc = gsr_batch_step(b, gs, sh, targets, lambda, grads, shape, sh_grad, nullptr, 1);
-      if (rc) return rc;
-    }
-    double dt;
-    if ((rc = time_min(4, dt))) return rc;
-    if (dt < best_first_t) {
-      best_first_t = dt;
-      best_first = sf;
-    }
-  }
-  b->split_first = best_first;
-  drop_graph(b);
-  if (tile_batch) *tile_batch = best_tb;
-  if (chunk) *chunk = best_ch;
+  // so their list lengths say little); time_min's warm step fills that estimate.
+  const std::vector<uint32_t> firsts = values(b->split_first, {0u, 1024u, 2048u});
+  if ((rc = pick(firsts.size(),
+                 [&](size_t i) { return gsr_batch_set_split(b, firsts[i], b->split_later, b->split_backward); }, k)))
+    return rc;
+  if (tile_batch) *tile_batch = b->tile_batch;
+  if (chunk) *chunk = b->chunk;
   return GSR_OK;
 }
 

@@ -18,9 +18,23 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
                                                     int deg, uint32_t n, float* __restrict__ grads,
                                                     uint64_t stride, float* __restrict__ shape,
                                                     float* __restrict__ sh_grad, float* __restrict__ grads_out,
-                                                    int consume, uint32_t* __restrict__ touched, uint32_t record) {
+                                                    int consume, uint32_t* __restrict__ touched, uint32_t record,
+                                                    const uint64_t* __restrict__ valid_mask,
+                                                    float* __restrict__ stats,
+                                                    const uint32_t* __restrict__ stats_guard) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // A step whose entries overflowed their storage is run again: its statistics
+  // are left out, so they are added once.
+  if (stats && stats_guard && *stats_guard) stats = nullptr;
+  // STATS (gpu.js:491-497) per view, in view order: the running sums start from
+  // the stored ones, so V views add exactly as V single-view STATS passes would.
+  float st_norm = 0.0f, st_count = 0.0f;
+  if (stats) {
+    st_norm = stats[size_t(i) * 2];
+    st_count = stats[size_t(i) * 2 + 1];
+  }
+  const size_t mask_words = valid_mask_words(n);
   const float4* g4 = reinterpret_cast<const float4*>(gs) + size_t(i) * 4;
   const float4 pa = g4[0], pb = g4[1], pc = g4[2];
   const float mean[3] = {pa.x, pa.y, pa.z};
@@ -178,8 +192,18 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
       rec[k][8] = reinterpret_cast<const float*>(r)[8];
     }
 #pragma unroll
-    for (uint32_t k = 0; k < kAhead; ++k)
-      if (v0 + k < num_views) view_step(v0 + k, rec[k]);
+    for (uint32_t k = 0; k < kAhead; ++k) {
+      if (v0 + k >= num_views) continue;
+      if (stats && ((valid_mask[size_t(v0 + k) * mask_words + i / 64] >> (i % 64)) & 1u)) {
+        st_norm += sqrtf(rec[k][4] * rec[k][4] + rec[k][5] * rec[k][5]);
+        st_count += 1.0f;
+      }
+      view_step(v0 + k, rec[k]);
+    }
+  }
+  if (stats) {
+    stats[size_t(i) * 2] = st_norm;
+    stats[size_t(i) * 2 + 1] = st_count;
   }
   // Sigma = M M^T, M = R diag(s): dL/dM = 2 Gw M; log-scale and rotation.
   const float s[3] = {expf(q[3]), expf(q[4]), expf(q[5])};
@@ -353,10 +377,13 @@ __global__ __launch_bounds__(256) void stats_kernel(uint32_t n, const float* __r
 
 void launch_shape(hipStream_t s, const gsr_camera* cams, uint32_t num_views, const float* gs, const float* sh,
                   int sh_degree, uint32_t n, float* grads, uint64_t grads_view_stride, float* shape,
-                  float* sh_grad, float* grads_out, int consume, uint32_t* touched, uint32_t record) {
+                  float* sh_grad, float* grads_out, int consume, uint32_t* touched, uint32_t record,
+                  const uint64_t* valid_mask, float* stats, const uint32_t* stats_guard) {
   if (n == 0) return;
+  if (!valid_mask) stats = nullptr;
   hipLaunchKernelGGL(shape_kernel, dim3((n + 255) / 256), dim3(256), 0, s, cams, num_views, gs, sh, sh_degree, n,
-                     grads, grads_view_stride, shape, sh_grad, grads_out, consume, touched, record);
+                     grads, grads_view_stride, shape, sh_grad, grads_out, consume, touched, record, valid_mask,
+                     stats, stats_guard);
 }
 
 void launch_adam(hipStream_t s, const gsr_step& step, float* gs, const float* grads, const float* shape,

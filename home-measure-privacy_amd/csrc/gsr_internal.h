@@ -59,10 +59,16 @@ __host__ __device__ inline uint32_t work_last(const TileWork& w) { return (w.vie
 // traffic of the sort and the binning.
 constexpr uint32_t kPackedRectTiles = 256;
 inline bool rect_packed(uint32_t max_tiles_axis) { return max_tiles_axis <= kPackedRectTiles; }
-// splats [V][n][12]; keys/rects/counts optional [V][n] (binning inputs).
+// splats [V][n][12]; keys/rects/counts optional [V][n] (binning inputs).  With
+// hitbox the rectangles cover the tiles of each splat's hit box (where alpha can
+// reach 1/255) inside the reference square, else the square itself (tileLists'
+// rectangle).  valid_mask (optional, [V][ceil(n / 64)]): bit i of word
+// [v][i / 64] set when Gaussian i projects validly in view v.
 void launch_project(hipStream_t s, const gsr_camera* cams, const ViewDesc* views, uint32_t num_views,
                     const float* gs, const float* sh, int sh_degree, uint32_t n, float* splats, uint32_t* keys,
-                    uint2* rects, uint32_t* counts, bool packed = false);
+                    uint2* rects, uint32_t* counts, bool packed = false, bool hitbox = true,
+                    uint64_t* valid_mask = nullptr);
+__host__ __device__ inline size_t valid_mask_words(uint32_t n) { return (size_t(n) + 63) / 64; }
 // Binning inputs from existing splat records (drop-in path).
 void launch_prep(hipStream_t s, const ViewDesc* views, uint32_t num_views, const float* splats, uint32_t n,
                  uint32_t* keys, uint2* rects, bool packed);
@@ -155,6 +161,9 @@ struct RasterArgs {
   uint8_t* band_open;    // null for a single pass
   uint32_t* walk;        // forward, optional: per global tile, entries examined in this pass (max over items)
   uint32_t* last;        // forward, optional: per global tile, furthest last hit - segment start (max over items)
+  // Nonzero: no hit-box culling.  Every entry is evaluated at every pixel of
+  // its tile, as alphaAt does; results must not change (verification knob).
+  int no_cull;
 };
 // done[g] = (no band of tile g open) with assign != 0 (after the first pass, which
 // blends every tile), else done[g] |= (no band open).
@@ -196,9 +205,15 @@ void launch_zero(hipStream_t s, void* p, size_t bytes);
 // touched (optional, [num_views]) counts the Gaussians with any gradient per view.
 // record: floats per per-view gradient record (16, or 12 for the batch's packed
 // records; the nine terms sit at 0-8 either way).
+// stats (optional, [n][2], with valid_mask from launch_project): STATS
+// (gpu.js:483-497) for every view in turn, stats[i] += (|(du, dv)|, 1) where
+// Gaussian i projects validly in the view; left unchanged when the device word
+// stats_guard (optional) is nonzero.
 void launch_shape(hipStream_t s, const gsr_camera* cams, uint32_t num_views, const float* gs, const float* sh,
                   int sh_degree, uint32_t n, float* grads, uint64_t grads_view_stride, float* shape,
-                  float* sh_grad, float* grads_out, int consume, uint32_t* touched = nullptr, uint32_t record = 16);
+                  float* sh_grad, float* grads_out, int consume, uint32_t* touched = nullptr, uint32_t record = 16,
+                  const uint64_t* valid_mask = nullptr, float* stats = nullptr,
+                  const uint32_t* stats_guard = nullptr);
 void launch_adam_sh(hipStream_t s, const gsr_step& step, float lr_sh_rest, float* gs, float* sh, const float* grads,
                     const float* shape, const float* sh_grad, float* moments);
 void launch_adam(hipStream_t s, const gsr_step& step, float* gs, const float* grads, const float* shape,

@@ -80,9 +80,11 @@ __global__ __launch_bounds__(256) void project_kernel(const gsr_camera* __restri
                                                       const float* __restrict__ gs, const float* __restrict__ sh,
                                                       int deg, uint32_t n, float* __restrict__ splats,
                                                       uint32_t* __restrict__ keys, uint2* __restrict__ rects,
-                                                      uint32_t* __restrict__ counts, bool packed) {
+                                                      uint32_t* __restrict__ counts, bool packed, bool hitbox,
+                                                      uint64_t* __restrict__ valid_mask) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const size_t mask_words = valid_mask_words(n);
   GaussLocal g;
   load_gaussian(gs, sh, deg, i, g);
   for (uint32_t v = 0; v < num_views; ++v) {
@@ -92,13 +94,20 @@ __global__ __launch_bounds__(256) void project_kernel(const gsr_camera* __restri
     if (deg > 0 && p.ok) sh_colour(deg, g.sh, g.mean, cam, rgb);
     size_t idx = size_t(v) * n + i;
     write_splat(splats, idx, p, g.opacity, rgb);
+    if (valid_mask) {
+      // Waves cover 64 consecutive Gaussians from a multiple of 64 (256-thread
+      // blocks), so one ballot is one mask word; lanes past n are inactive.
+      const uint64_t m = __ballot(p.ok);
+      if (__lane_id() == 0) valid_mask[size_t(v) * mask_words + i / 64] = m;
+    }
     if (keys || rects || counts) {
       // The batched step bins each splat to the tiles of its hit box (where
       // alpha can reach 1/255, padded by half a pixel for the float32 pixel
       // offsets) within the reference square: the tiles it leaves out hold no
       // pixel the splat can hit, so renders and gradients are unchanged.
+      // Without hitbox: the reference square, as tileLists bins.
       float rx = p.radius, ry = p.radius;
-      if (p.ok && fabsf(p.u) < 0x1p20f && fabsf(p.v) < 0x1p20f) {
+      if (hitbox && p.ok && fabsf(p.u) < 0x1p20f && fabsf(p.v) < 0x1p20f) {
         const float2 e = hit_extent(p.a, p.b, p.c, g.opacity, p.radius);
         rx = e.x < 0.0f ? e.x : fminf(p.radius, e.x + 0.5f);
         ry = e.y < 0.0f ? e.y : fminf(p.radius, e.y + 0.5f);
@@ -130,11 +139,11 @@ __global__ __launch_bounds__(256) void prep_kernel(const ViewDesc* __restrict__ 
 
 void launch_project(hipStream_t s, const gsr_camera* cams, const ViewDesc* views, uint32_t num_views,
                     const float* gs, const float* sh, int sh_degree, uint32_t n, float* splats, uint32_t* keys,
-                    uint2* rects, uint32_t* counts, bool packed) {
+                    uint2* rects, uint32_t* counts, bool packed, bool hitbox, uint64_t* valid_mask) {
   if (n == 0 || num_views == 0) return;
   dim3 grid((n + 255) / 256);
   hipLaunchKernelGGL(project_kernel, grid, dim3(256), 0, s, cams, views, num_views, gs, sh, sh_degree, n, splats,
-                     keys, rects, counts, packed);
+                     keys, rects, counts, packed, hitbox, valid_mask);
 }
 
 void launch_prep(hipStream_t s, const ViewDesc* views, uint32_t num_views, const float* splats, uint32_t n,

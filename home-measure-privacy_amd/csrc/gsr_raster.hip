@@ -99,8 +99,13 @@ __device__ inline Batch fetch_batch(const Tile& t, uint32_t k0, uint32_t lane) {
   return b;
 }
 
+// The entry's hit box, or with no_cull an unbounded one: every pixel of the tile
+// is then evaluated, as alphaAt (gpu.js:91-103) does.  An invalid splat misses
+// everywhere either way (gpu.js:93).
+template <bool kNoCull>
 __device__ inline float2 entry_extent(const Batch& b) {
   if (b.r2.w < 0.5f) return make_float2(kNever, kNever);  // invalid splat
+  if (kNoCull) return make_float2(__builtin_huge_valf(), __builtin_huge_valf());
   return hit_extent(b.r0.z, b.r0.w, b.r1.x, b.r1.y, b.r1.z);
 }
 
@@ -212,9 +217,9 @@ __device__ inline bool band_culled(float ylo, float v, float ey) {
 // ..) that the hit box reaches.  A band whose bit is clear holds no pixel the
 // entry can hit.  (The forward recomputes these in its walk: there the extra
 // LDS read per entry costs more than it saves.)
-template <int kQuads>
+template <int kQuads, bool kNoCull>
 __device__ inline void stage(float4 (*lds)[kQuads], uint32_t lane, const Batch& b, float band0) {
-  const float2 e = entry_extent(b);
+  const float2 e = entry_extent<kNoCull>(b);
   lds[lane][0] = b.r0;
   lds[lane][1] = make_float4(b.r1.x, b.r1.y, b.r1.z, e.x);
   lds[lane][2] = make_float4(b.r2.x, b.r2.y, b.r2.z, e.y);
@@ -241,9 +246,11 @@ __device__ inline uint32_t entry_bands(const float4 (*lds)[4], uint32_t j, uint3
 // ---- forward -------------------------------------------------------------------
 
 // kLast: also track each pixel's last hit (the optional state output and the
-// profiling counters need it; the batched training step does not).
+// profiling counters need it; the batched training step does not).  kNoCull:
+// evaluate every entry at every pixel (RasterArgs::no_cull), a kernel of its own
+// so the regular blend keeps its register budget.
 // Inlined into the kernels, so the kernel arguments are read in place.
-template <bool kLast>
+template <bool kLast, bool kNoCull>
 __device__ __forceinline__ void forward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[3]) {
   const uint32_t lane = __lane_id();
   Tile t;
@@ -281,7 +288,7 @@ __device__ __forceinline__ void forward_tile(const RasterArgs& a, uint32_t w, fl
       break;
     }
     __syncthreads();
-    stage<3>(lds, lane, nb, band0);
+    stage<3, kNoCull>(lds, lane, nb, band0);
     __syncthreads();
     if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
     const uint32_t nk = min(64u, t.end - k0);
@@ -362,10 +369,10 @@ __device__ __forceinline__ void forward_tile(const RasterArgs& a, uint32_t w, fl
 
 // kLast (the optional state output and the profiling counters) is a kernel of
 // its own, so the regular blend keeps its smaller register budget.
-template <bool kLast>
+template <bool kLast, bool kNoCull>
 __global__ __launch_bounds__(64) void forward_kernel(RasterArgs a) {
   __shared__ float4 lds[64][3];
-  forward_tile<kLast>(a, blockIdx.x, lds);
+  forward_tile<kLast, kNoCull>(a, blockIdx.x, lds);
 }
 
 __device__ inline uint32_t pop_work(uint32_t* queue, uint32_t n) {
@@ -406,10 +413,10 @@ __device__ __forceinline__ void for_each_item(const RasterArgs& a, Blend&& blend
   }
 }
 
-template <bool kLast>
+template <bool kLast, bool kNoCull>
 __global__ __launch_bounds__(64) void forward_persistent_kernel(RasterArgs a) {
   __shared__ float4 lds[64][3];
-  for_each_item(a, [&](uint32_t w) __attribute__((always_inline)) { forward_tile<kLast>(a, w, lds); });
+  for_each_item(a, [&](uint32_t w) __attribute__((always_inline)) { forward_tile<kLast, kNoCull>(a, w, lds); });
 }
 
 // ---- backward ------------------------------------------------------------------
@@ -466,6 +473,7 @@ __device__ inline int reduce9_slot(uint32_t lane) {
 
 __device__ inline float sign_f(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
 
+template <bool kNoCull>
 __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[4], uint32_t* gi_lds) {
   const uint32_t lane = __lane_id();
   Tile t;
@@ -514,7 +522,7 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
   for (uint32_t k0 = t.start; k0 < t.end; k0 += 64) {
     if (!any_active(T)) break;
     __syncthreads();
-    stage<4>(lds, lane, nb, band0);
+    stage<4, kNoCull>(lds, lane, nb, band0);
     gi_lds[lane] = nb.gi;
     __syncthreads();
     if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
@@ -586,16 +594,18 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
   }
 }
 
+template <bool kNoCull>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void backward_kernel(RasterArgs a) {
   __shared__ float4 lds[64][4];
   __shared__ uint32_t gi_lds[64];
-  backward_tile(a, blockIdx.x, lds, gi_lds);
+  backward_tile<kNoCull>(a, blockIdx.x, lds, gi_lds);
 }
 
+template <bool kNoCull>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void backward_persistent_kernel(RasterArgs a) {
   __shared__ float4 lds[64][4];
   __shared__ uint32_t gi_lds[64];
-  for_each_item(a, [&](uint32_t w) __attribute__((always_inline)) { backward_tile(a, w, lds, gi_lds); });
+  for_each_item(a, [&](uint32_t w) __attribute__((always_inline)) { backward_tile<kNoCull>(a, w, lds, gi_lds); });
 }
 
 // ---- loss gradient (gpu.js:336-409) ---------------------------------------------
@@ -886,30 +896,45 @@ void launch_done_update(hipStream_t s, uint32_t total_tiles, const uint8_t* band
                      reinterpret_cast<const uint32_t*>(band_open), done, assign);
 }
 
+template <bool kLast, bool kNoCull>
+void launch_forward_variant(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid) {
+  if (persistent)
+    hipLaunchKernelGGL((forward_persistent_kernel<kLast, kNoCull>), dim3(grid), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((forward_kernel<kLast, kNoCull>), dim3(a.num_work), dim3(64), 0, s, a);
+}
+
 void launch_forward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid) {
   if (a.num_work == 0) return;
   const bool track = a.state || a.last;
-  if (persistent) {
-    if (track) hipLaunchKernelGGL(forward_persistent_kernel<true>, dim3(grid), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL(forward_persistent_kernel<false>, dim3(grid), dim3(64), 0, s, a);
+  if (a.no_cull) {
+    if (track) launch_forward_variant<true, true>(s, a, persistent, grid);
+    else launch_forward_variant<false, true>(s, a, persistent, grid);
   } else {
-    if (track) hipLaunchKernelGGL(forward_kernel<true>, dim3(a.num_work), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL(forward_kernel<false>, dim3(a.num_work), dim3(64), 0, s, a);
+    if (track) launch_forward_variant<true, false>(s, a, persistent, grid);
+    else launch_forward_variant<false, false>(s, a, persistent, grid);
   }
+}
+
+template <bool kNoCull>
+void launch_backward_variant(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid) {
+  if (persistent)
+    hipLaunchKernelGGL(backward_persistent_kernel<kNoCull>, dim3(grid), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL(backward_kernel<kNoCull>, dim3(a.num_work), dim3(64), 0, s, a);
 }
 
 void launch_backward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid) {
   if (a.num_work == 0) return;
-  if (persistent)
-    hipLaunchKernelGGL(backward_persistent_kernel, dim3(grid), dim3(64), 0, s, a);
-  else
-    hipLaunchKernelGGL(backward_kernel, dim3(a.num_work), dim3(64), 0, s, a);
+  if (a.no_cull) launch_backward_variant<true>(s, a, persistent, grid);
+  else launch_backward_variant<false>(s, a, persistent, grid);
 }
 
 void raster_occupancy(int* forward_per_cu, int* backward_per_cu) {
   int f = 0, b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&f, forward_persistent_kernel<false>, 64, 0) != hipSuccess) f = 8;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, backward_persistent_kernel, 64, 0) != hipSuccess) b = 8;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&f, forward_persistent_kernel<false, false>, 64, 0) != hipSuccess)
+    f = 8;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, backward_persistent_kernel<false>, 64, 0) != hipSuccess) b = 8;
   *forward_per_cu = f;
   *backward_per_cu = b;
 }

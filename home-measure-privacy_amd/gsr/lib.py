@@ -25,7 +25,8 @@ EXPORTS = (
     "gsr_shape", "gsr_adam", "gsr_adam_sh", "gsr_stats", "gsr_batch_create", "gsr_batch_destroy", "gsr_batch_set_cameras",
     "gsr_batch_step", "gsr_batch_entries", "gsr_batch_configure", "gsr_batch_autotune", "gsr_batch_profile",
     "gsr_batch_set_passes", "gsr_loss_grad_window", "gsr_batch_set_ssim_window", "gsr_batch_set_split",
-    "gsr_batch_tuning", "gsr_batch_set_emit_band",
+    "gsr_batch_tuning", "gsr_batch_set_emit_band", "gsr_set_culling", "gsr_batch_step_stats", "gsr_batch_set_culling",
+    "gsr_batch_tile_lists",
 )
 SSIM_BOX7, SSIM_GAUSS11 = 7, 11
 STAGES = ("project", "depth_sort", "bin_count", "bin_emit", "forward", "loss_grad", "backward", "chain")
@@ -97,6 +98,10 @@ def load():
         "gsr_batch_tuning": (i32, [P, ctypes.POINTER(u32)]),
         "gsr_batch_set_emit_band": (i32, [P, u32]),
         "gsr_batch_set_split": (i32, [P, u32, u32, u32]),
+        "gsr_set_culling": (i32, [P, i32]),
+        "gsr_batch_step_stats": (i32, [P, P, P, P, f32, P, P, P, P, P, i32]),
+        "gsr_batch_set_culling": (i32, [P, i32]),
+        "gsr_batch_tile_lists": (i32, [P, u32, u32, P, u64, P, ctypes.POINTER(u64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -192,6 +197,10 @@ class Context:
     def sync(self):
         self._check(self.lib.gsr_synchronize(self.h))
 
+    def set_culling(self, enabled: bool):
+        """Hit-box culling of forward / backward (a verification knob: results do not change)."""
+        self._check(self.lib.gsr_set_culling(self.h, 1 if enabled else 0))
+
     # ---- operators
     def project(self, cam, gs: Buffer, n: int, splats: Buffer, sh: Buffer | None = None, deg: int = 0):
         c = camera(cam, n)
@@ -259,6 +268,7 @@ class Batch:
         self.ctx, self.lib = ctx, ctx.lib
         self.V, self.n, self.deg = len(cams), n, deg
         arr = (Camera * len(cams))(*[camera(c, n) for c in cams])
+        self._tiles = [((c.width + 15) // 16) * ((c.height + 15) // 16) for c in cams]
         h = P()
         ctx._check(self.lib.gsr_batch_create(ctx.h, len(cams), arr, n, deg, ctypes.byref(h)))
         self.h = h
@@ -273,10 +283,38 @@ class Batch:
         self.ctx._check(self.lib.gsr_batch_set_cameras(self.h, arr))
 
     def step(self, gs: Buffer, sh: Buffer | None, targets: Buffer, lam: float, grads: Buffer, shape: Buffer,
-             sh_grad: Buffer | None = None, images: Buffer | None = None, graph: bool = True):
-        self.ctx._check(self.lib.gsr_batch_step(self.h, gs.p, sh.p if sh else None, targets.p, lam, grads.p, shape.p,
-                                                sh_grad.p if sh_grad else None, images.p if images else None,
-                                                1 if graph else 0))
+             sh_grad: Buffer | None = None, images: Buffer | None = None, graph: bool = True,
+             stats: Buffer | None = None):
+        """One forward + backward of every view; with ``stats`` ([n][2] floats)
+        also adds each view's STATS (|(du, dv)|, 1) for validly projected Gaussians."""
+        if stats is None:
+            self.ctx._check(self.lib.gsr_batch_step(self.h, gs.p, sh.p if sh else None, targets.p, lam, grads.p,
+                                                    shape.p, sh_grad.p if sh_grad else None,
+                                                    images.p if images else None, 1 if graph else 0))
+        else:
+            self.ctx._check(self.lib.gsr_batch_step_stats(self.h, gs.p, sh.p if sh else None, targets.p, lam,
+                                                          grads.p, shape.p, sh_grad.p if sh_grad else None, stats.p,
+                                                          images.p if images else None, 1 if graph else 0))
+
+    def set_culling(self, enabled: bool):
+        """Hit-box binning and culling (a verification knob: results do not change)."""
+        self.ctx._check(self.lib.gsr_batch_set_culling(self.h, 1 if enabled else 0))
+
+    def tile_lists(self, view: int, pass_: int = 0):
+        """The last step's (order, ranges) of one view and depth slice, as host arrays."""
+        e = ctypes.c_uint64(0)
+        rb = self.ctx.alloc(8 * self._tiles[view])
+        rc = self.lib.gsr_batch_tile_lists(self.h, view, pass_, None, 0, rb.p, ctypes.byref(e))
+        if rc not in (GSR_OK, GSR_E_CAPACITY):
+            rb.free()
+            self.ctx._check(rc)
+        ob = self.ctx.alloc(max(1, e.value) * 4)
+        self.ctx._check(self.lib.gsr_batch_tile_lists(self.h, view, pass_, ob.p, e.value, rb.p, ctypes.byref(e)))
+        order = self.ctx.download(ob, np.uint32, (e.value,))
+        ranges = self.ctx.download(rb, np.uint32, (2 * self._tiles[view],))
+        ob.free()
+        rb.free()
+        return order, ranges
 
     def entries(self) -> np.ndarray:
         out = (ctypes.c_uint64 * self.V)()

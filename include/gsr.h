@@ -89,6 +89,13 @@ int gsr_clear(gsr_ctx* ctx, void* dptr, size_t bytes);
  * (gpu.js:719); pointers and bytes are multiples of 4.  Queued on the stream. */
 int gsr_copy(gsr_ctx* ctx, void* dst, const void* src, size_t bytes);
 int gsr_synchronize(gsr_ctx* ctx);
+/* Hit-box culling of gsr_forward / gsr_backward on ctx (default enabled).  No
+ * counterpart in the reference: the blends skip entries and 4-row bands whose
+ * conservative hit box (where alpha can reach 1/255) misses the pixels still
+ * open, which alphaAt (gpu.js:91-103) would evaluate to a miss.  Disabled, every
+ * entry is evaluated at every pixel of its tile; images, states and gradients
+ * are the same (verification knob). */
+int gsr_set_culling(gsr_ctx* ctx, int enabled);
 
 /* ---- operators (one per pipeline of gpu.js:712) */
 
@@ -185,8 +192,27 @@ int gsr_batch_set_cameras(gsr_batch* batch, const gsr_camera* cams);
  * the pointer arguments stay the same. */
 int gsr_batch_step(gsr_batch* batch, const float* gs, const float* sh, const float* targets, float lambda,
                    float* grads, float* shape, float* sh_grad, float* images, int use_graph);
+/* gsr_batch_step that also accumulates the densify statistics of STATS
+ * (gpu.js:483-497) for every view in turn: stats[n][2] += (|(du, dv)|, 1) with
+ * the view's screen-position gradient, for each Gaussian that projects validly
+ * in the view (the sum of one STATS pass per view).  stats may be NULL. */
+int gsr_batch_step_stats(gsr_batch* batch, const float* gs, const float* sh, const float* targets, float lambda,
+                         float* grads, float* shape, float* sh_grad, float* stats, float* images, int use_graph);
 /* Per-view entry counts of the last step (host array of num_views). */
 int gsr_batch_entries(gsr_batch* batch, uint64_t* entries);
+/* Hit-box binning and culling of the batched step (default enabled).  Disabled,
+ * each splat is binned to the reference square's tiles (tileLists, gpu.js:455-
+ * 462) and the blends evaluate every entry at every pixel; results are the same
+ * (verification knob; no counterpart in the reference). */
+int gsr_batch_set_culling(gsr_batch* batch, int enabled);
+/* The last step's tile lists of one view and depth slice (inspection; no
+ * counterpart in the reference): order (device, order_capacity slots) receives
+ * the slice's entries of the view, tile-major in (depth, index) order, and
+ * ranges (device, 2 * tiles of the view) their [start, end) per tile, as
+ * gsr_bin_sort writes them.  *num_entries is set; GSR_E_CAPACITY when order is
+ * too small (ranges are written). */
+int gsr_batch_tile_lists(gsr_batch* batch, uint32_t view, uint32_t pass, uint32_t* order, uint64_t order_capacity,
+                         uint32_t* ranges, uint64_t* num_entries);
 /* Tuning knobs: tiles claimed per queue pop (0 = keep) and Gaussians per
  * binning chunk (0 = keep).  gsr_batch_autotune times candidates on the given
  * inputs and keeps the fastest; it writes the choice to *tile_batch / *chunk.

@@ -349,6 +349,36 @@ inline bool alpha_at(const R* o, R px, R py, R& alpha, R& dx, R& dy) {
   return true;
 }
 
+// For each (tile, splat) pair: the largest opacity * exp(power) of alphaAt
+// (gpu.js:92-103, clamped to 0.99 as there) over the tile's pixels that pass its
+// square and power <= 0 tests, or 0 when none does.  Below 1/255 the splat hits
+// no pixel of the tile.  Float64 from the float32 records.
+void tile_reach(const float* splats, uint64_t pairs, const uint32_t* tiles, const uint32_t* gidx, int W, int H,
+                double* out) {
+  const int tilesX = (W + 15) / 16;
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (long k = 0; k < long(pairs); ++k) {
+    const float* o = splats + size_t(gidx[k]) * 12;
+    const int tx = int(tiles[k] % uint32_t(tilesX)), ty = int(tiles[k] / uint32_t(tilesX));
+    const double u = o[0], v = o[1], A = o[2], B = o[3], C = o[4], op = o[5], r = o[6];
+    double best = -1e300;
+    bool any = false;
+    for (int y = ty * 16; y < std::min(H, ty * 16 + 16); ++y) {
+      const double dy = (double(y) + 0.5) - v;
+      if (std::fabs(dy) > r) continue;
+      for (int x = tx * 16; x < std::min(W, tx * 16 + 16); ++x) {
+        const double dx = (double(x) + 0.5) - u;
+        if (std::fabs(dx) > r) continue;
+        const double power = -0.5 * (A * dx * dx + C * dy * dy) - B * dx * dy;
+        if (power > 0.0) continue;
+        if (!any || power > best) best = power;
+        any = true;
+      }
+    }
+    out[k] = (any && o[11] >= 0.5f) ? std::min(0.99, op * std::exp(best)) : 0.0;
+  }
+}
+
 // FORWARD.main, gpu.js:113-137.  image HWC RGB, state (T, last) per pixel.
 template <class R>
 void forward(const R* splats, const uint32_t* list, const uint32_t* ranges, int W, int H, R* image, R* state) {
@@ -908,5 +938,10 @@ int oracle_num_threads(int n) {
 
 GSR_ORACLE_EXPORTS(float, f32)
 GSR_ORACLE_EXPORTS(double, f64)
+
+void oracle_tile_reach(const float* splats, uint64_t pairs, const uint32_t* tiles, const uint32_t* gidx, int W, int H,
+                       double* out) {
+  tile_reach(splats, pairs, tiles, gidx, W, H, out);
+}
 
 }  // extern "C"

@@ -200,6 +200,19 @@ def stats(grads, splats, st, dtype=np.float32):
     return st
 
 
+def tile_reach(splats, tiles, gidx, width: int, height: int) -> np.ndarray:
+    """Per (tile, Gaussian) pair: the largest alpha of alphaAt (gpu.js:92-103) over
+    the tile's pixels in float64 (0 when the square or the power test rejects them
+    all); below 1/255 the splat hits no pixel of the tile."""
+    splats = np.ascontiguousarray(splats, dtype=np.float32)
+    tiles = np.ascontiguousarray(tiles, dtype=np.uint32)
+    gidx = np.ascontiguousarray(gidx, dtype=np.uint32)
+    out = np.zeros(tiles.size, dtype=np.float64)
+    lib().oracle_tile_reach(_p(splats), ctypes.c_uint64(tiles.size), _p(tiles), _p(gidx), ctypes.c_int(width),
+                            ctypes.c_int(height), _p(out))
+    return out
+
+
 def sh_basis(deg: int, d, dtype=np.float64):
     d = _c(d, dtype)
     Y = np.zeros(16, dtype=dtype)

@@ -519,3 +519,45 @@ def test_stats_accumulate():
     st = O.stats(grads, sp, np.zeros((3, 2), np.float32))
     st = O.stats(grads, sp, st)
     np.testing.assert_allclose(st, [[10, 2], [0, 0], [4, 2]])
+
+
+def test_tile_reach_matches_single_splat_renders():
+    """tile_reach (the largest alpha a splat reaches over a tile) against the
+    oracle's own forward: each splat rendered alone into each tile of a small
+    image (one-entry lists, T = 1, white colour), so a pixel's red channel is the
+    splat's alpha there, or 0 where alphaAt misses.  Agreement where the forward
+    hits; where tile_reach is below 1/255 no pixel of the tile is hit."""
+    rng = np.random.default_rng(17)
+    W, H, n = 67, 41, 300  # ragged tiles on both edges
+    sp = np.zeros((n, 12), np.float32)
+    sp[:, 0] = rng.uniform(-10, W + 10, n)
+    sp[:, 1] = rng.uniform(-10, H + 10, n)
+    a = rng.uniform(0.002, 0.3, n)
+    c = rng.uniform(0.002, 0.3, n)
+    b = rng.uniform(-0.9, 0.9, n) * np.sqrt(a * c)
+    sp[:, 2], sp[:, 3], sp[:, 4] = a, b, c
+    sp[:, 5] = rng.uniform(0.002, 1.0, n)
+    sp[:, 6] = np.ceil(rng.uniform(1, 30, n))
+    sp[:, 7] = 1.0
+    sp[:, 8:11] = 1.0
+    sp[:, 11] = 1.0
+    tx, ty = (W + 15) // 16, (H + 15) // 16
+    T = tx * ty
+    tiles = np.repeat(np.arange(T, dtype=np.uint32), n)
+    gidx = np.tile(np.arange(n, dtype=np.uint32), T)
+    reach = O.tile_reach(sp, tiles, gidx, W, H).reshape(T, n)
+    hits = misses = 0
+    for i in range(n):
+        lst = np.array([i] * T, dtype=np.uint32)
+        ranges = np.stack([np.arange(T), np.arange(T) + 1], 1).astype(np.uint32).ravel()
+        img, _ = O.forward(sp.astype(np.float64), lst, ranges, W, H)
+        for t in range(T):
+            y0, x0 = (t // tx) * 16, (t % tx) * 16
+            peak = img[y0:y0 + 16, x0:x0 + 16, 0].max()
+            if reach[t, i] >= 1 / 255:
+                assert peak == pytest.approx(reach[t, i], rel=1e-12)
+                hits += 1
+            else:
+                assert peak == 0.0
+                misses += 1
+    assert hits > 100 and misses > 100
