@@ -233,6 +233,34 @@ def copy_calibration(mib: int):
     ctx.close()
 
 
+def valu_peak():
+    """The sustained VALU issue rate of the chip: independent non-packed fma chains
+    (csrc/calib) on 8 blocks of 4 waves per CU.  One JSON line."""
+    import ctypes
+
+    ctx = lib.Context(0)
+    cal = ctypes.CDLL(os.path.join(ROOT, "home-measure-privacy_amd", "gsr", "libgsr_calib.so"))
+    cus, khz = ctypes.c_int(0), ctypes.c_int(0)
+    assert cal.gsr_calib_device(ctypes.byref(cus), ctypes.byref(khz)) == 0
+    blocks, trips = cus.value * 8, 16384
+    out = ctx.alloc(blocks * 256 * 4)
+    ms, fma = ctypes.c_float(0), ctypes.c_uint64(0)
+    rc = cal.gsr_calib_valu(ctypes.c_void_p(out.ptr), ctypes.c_uint32(blocks), ctypes.c_uint32(trips),
+                            ctypes.byref(ms), ctypes.byref(fma))
+    assert rc == 0, rc
+    waves = blocks * 4
+    instr = waves * fma.value  # wave-level v_fma_f32 instructions
+    rate = instr / (ms.value * 1e-3)
+    simd_cycles = cus.value * 4 * khz.value * 1e3
+    print(json.dumps({"kernel": "valu_peak_kernel", "cus": cus.value, "peak_clock_mhz": khz.value / 1e3,
+                      "waves": waves, "valu_wave_instructions": instr, "ms": round(ms.value, 4),
+                      "valu_wave_instructions_per_s": rate,
+                      "per_simd_per_cycle_at_peak_clock": round(rate / simd_cycles, 4),
+                      "fp32_tflops_nonpacked": round(rate * 64 * 2 / 1e12, 2)}), flush=True)
+    out.free()
+    ctx.close()
+
+
 def tune(args, batch, gsb, shb, tgt, grads, shape, shg):
     """Sets the batch's knobs from --tune, or times candidates (autotune)."""
     if args.tune:
@@ -335,6 +363,7 @@ def main():
                     "to set instead of autotuning (the 'tuning' line of an earlier run)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0, help="only time a device copy of this many MiB")
+    ap.add_argument("--valu-peak", action="store_true", help="only measure the sustained VALU issue rate")
     ap.add_argument("--train", action="store_true", help="the 3M-Gaussian, 64-view training loop instead")
     ap.add_argument("--train-steps", type=int, default=200)
     args = ap.parse_args()
@@ -344,6 +373,9 @@ def main():
         sys.exit(2)
     if args.copy_mib:
         copy_calibration(args.copy_mib)
+        return
+    if args.valu_peak:
+        valu_peak()
         return
     if args.train:
         train_loop(args)

@@ -60,7 +60,25 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_calib(verbose)
     return LIB
+
+
+CALIB = os.path.join(PKG, "libgsr_calib.so")
+
+
+def build_calib(verbose: bool = True) -> str:
+    """The bench's measurement kernels (csrc/calib), a library of their own."""
+    src = os.path.join(CSRC, "calib", "gsr_calib.hip")
+    tmp = CALIB + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, "-shared", src, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, CALIB)
+    if verbose:
+        print(f"built {CALIB}")
+    return CALIB
 
 
 if __name__ == "__main__":
