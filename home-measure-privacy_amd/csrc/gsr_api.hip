@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -27,15 +28,49 @@ struct gsr_ctx {
   uint64_t* pinned = nullptr;
   // Verification knob of gsr_forward / gsr_backward: no hit-box culling.
   int no_cull = 0;
+  // Device address of the static scratch on this context's device (null when
+  // the lookup failed; scratch_error says why).
+  void* scratch = nullptr;
+  const char* scratch_error = "";
 };
 
 namespace gsr {
 hipError_t ctx_malloc(gsr_ctx* ctx, void** p, size_t bytes);
 }  // namespace gsr
 
+// Scratch of gsr_bin_sort_noalloc: a file-scope device array, so that call
+// allocates nothing.  Every GPU the library runs on holds its own copy until
+// the process ends; calls on one GPU share it (and race if they overlap).
+constexpr size_t kScratchBytes = size_t(48) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 thread_local std::string g_create_error;
+
+// The scratch's device address per device, looked up once per device with that
+// device current (host code must not use the array's name as an address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+void lookup_scratch(gsr_ctx* ctx) {
+  if (ctx->device >= kMaxDevices) {
+    ctx->scratch_error = "device index above the scratch table";
+    return;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_done[ctx->device]) {
+    void* p = nullptr;
+    g_scratch_err[ctx->device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_ptr[ctx->device] = g_scratch_err[ctx->device] == hipSuccess ? p : nullptr;
+    g_scratch_done[ctx->device] = true;
+  }
+  ctx->scratch = g_scratch_ptr[ctx->device];
+  if (!ctx->scratch) ctx->scratch_error = hipGetErrorString(g_scratch_err[ctx->device]);
+}
 
 int fail(gsr_ctx* ctx, int code, const char* fmt, ...) {
   char buf[512];
@@ -167,6 +202,7 @@ int gsr_create(int device, gsr_ctx** out) {
     delete ctx;
     return fail(nullptr, GSR_E_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
   }
+  lookup_scratch(ctx);  // device `device` is current
   *out = ctx;
   return GSR_OK;
 }
@@ -353,6 +389,53 @@ int gsr_bin_sort(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, uint3
       (rc = pooled_t(ctx, "entries", 1, &w.entries)) || (rc = pooled_t(ctx, "ebase", 1, &w.ebase)) ||
       (rc = pooled_t(ctx, "run", 1, &w.run)) || (rc = pooled_t(ctx, "overflow", 1, &w.overflow)))
     return rc;
+  return bin_sort_run(ctx, host, dview, splats, n, order, order_capacity, ranges, num_entries, w, z.chunk);
+}
+
+int gsr_bin_sort_noalloc(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, uint32_t n, uint32_t* order,
+                         uint64_t order_capacity, uint32_t* ranges, uint64_t* num_entries) {
+  if (!ctx) return GSR_E_INVALID;
+  int rc = check_bin_sort_args(ctx, splats, n, order, order_capacity, ranges, num_entries);
+  if (rc) return rc;
+  if ((rc = check_camera(ctx, cam))) return rc;
+  if (!ctx->scratch) return fail(ctx, GSR_E_HIP, "the static scratch is unavailable: %s", ctx->scratch_error);
+  const ViewDesc host = view_of(*cam, 0, 0, 0);
+  const BinSortSizes z = bin_sort_sizes(host, n);
+  // Regions of the device's static scratch, 256-byte aligned, in this order:
+  // offsets first, so the total is checked before any address is formed.
+  size_t at = 0;
+  auto take = [&](size_t bytes) {
+    const size_t off = at;
+    at += (bytes + 255) / 256 * 256;
+    return off;
+  };
+  const size_t o_view = take(sizeof(ViewDesc)), o_keys = take(z.nn * 4), o_keys_alt = take(z.nn * 4),
+               o_vals = take(z.nn * 4), o_vals_alt = take(z.nn * 4), o_rects = take(z.nn * 8),
+               o_rect_sorted = take(z.nn * 8), o_hist = take(z.hist * 4), o_piece = take(z.piece * 4),
+               o_matrix = take(z.matrix * 4), o_tile_tot = take(z.tiles * 4), o_entries = take(8),
+               o_ebase = take(8), o_run = take(8), o_overflow = take(4);
+  if (at > kScratchBytes)
+    return fail(ctx, GSR_E_CAPACITY,
+                "gsr_bin_sort_noalloc: %u Gaussians on a %ux%u view need %zu bytes of scratch, above its %zu",
+                n, host.width, host.height, at, kScratchBytes);
+  char* base = static_cast<char*>(ctx->scratch);
+  ViewDesc* dview = reinterpret_cast<ViewDesc*>(base + o_view);
+  BinSortScratch w;
+  w.keys = reinterpret_cast<uint32_t*>(base + o_keys);
+  w.keys_alt = reinterpret_cast<uint32_t*>(base + o_keys_alt);
+  w.vals = reinterpret_cast<uint32_t*>(base + o_vals);
+  w.vals_alt = reinterpret_cast<uint32_t*>(base + o_vals_alt);
+  w.rects = reinterpret_cast<uint2*>(base + o_rects);
+  w.rect_sorted = reinterpret_cast<uint2*>(base + o_rect_sorted);
+  w.hist = reinterpret_cast<uint32_t*>(base + o_hist);
+  w.piece = reinterpret_cast<uint32_t*>(base + o_piece);
+  w.matrix = reinterpret_cast<uint32_t*>(base + o_matrix);
+  w.tile_tot = reinterpret_cast<uint32_t*>(base + o_tile_tot);
+  w.entries = reinterpret_cast<uint64_t*>(base + o_entries);
+  w.ebase = reinterpret_cast<uint64_t*>(base + o_ebase);
+  w.run = reinterpret_cast<uint64_t*>(base + o_run);
+  w.overflow = reinterpret_cast<uint32_t*>(base + o_overflow);
+  GSR_HIP(ctx, hipMemcpyAsync(dview, &host, sizeof(ViewDesc), hipMemcpyHostToDevice, ctx->stream));
   return bin_sort_run(ctx, host, dview, splats, n, order, order_capacity, ranges, num_entries, w, z.chunk);
 }
 

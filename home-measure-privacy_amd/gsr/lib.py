@@ -26,7 +26,7 @@ EXPORTS = (
     "gsr_batch_step", "gsr_batch_entries", "gsr_batch_configure", "gsr_batch_autotune", "gsr_batch_profile",
     "gsr_batch_set_passes", "gsr_loss_grad_window", "gsr_batch_set_ssim_window", "gsr_batch_set_split",
     "gsr_batch_tuning", "gsr_batch_set_emit_band", "gsr_set_culling", "gsr_batch_step_stats", "gsr_batch_set_culling",
-    "gsr_batch_tile_lists",
+    "gsr_batch_tile_lists", "gsr_bin_sort_noalloc",
 )
 SSIM_BOX7, SSIM_GAUSS11 = 7, 11
 STAGES = ("project", "depth_sort", "bin_count", "bin_emit", "forward", "loss_grad", "backward", "chain")
@@ -102,6 +102,7 @@ def load():
         "gsr_batch_step_stats": (i32, [P, P, P, P, f32, P, P, P, P, P, i32]),
         "gsr_batch_set_culling": (i32, [P, i32]),
         "gsr_batch_tile_lists": (i32, [P, u32, u32, P, u64, P, ctypes.POINTER(u64)]),
+        "gsr_bin_sort_noalloc": (i32, [P, ctypes.POINTER(Camera), P, u32, P, u64, P, ctypes.POINTER(u64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -206,13 +207,15 @@ class Context:
         c = camera(cam, n)
         self._check(self.lib.gsr_project(self.h, ctypes.byref(c), gs.p, sh.p if sh else None, deg, n, splats.p))
 
-    def bin_sort(self, cam, splats: Buffer, n: int, order: Buffer | None, ranges: Buffer) -> int:
-        """Returns the entry count; raises GsrError(GSR_E_CAPACITY) when order is too small."""
+    def bin_sort(self, cam, splats: Buffer, n: int, order: Buffer | None, ranges: Buffer,
+                 noalloc: bool = False) -> int:
+        """Returns the entry count; raises GsrError(GSR_E_CAPACITY) when order is too small.
+        noalloc: gsr_bin_sort_noalloc (scratch in the library's static device array)."""
         c = camera(cam, n)
         e = ctypes.c_uint64(0)
         cap = 0 if order is None else order.nbytes // 4
-        rc = self.lib.gsr_bin_sort(self.h, ctypes.byref(c), splats.p, n, order.p if order else None, cap, ranges.p,
-                                   ctypes.byref(e))
+        fn = self.lib.gsr_bin_sort_noalloc if noalloc else self.lib.gsr_bin_sort
+        rc = fn(self.h, ctypes.byref(c), splats.p, n, order.p if order else None, cap, ranges.p, ctypes.byref(e))
         if rc == GSR_E_CAPACITY:
             err = GsrError(rc, self.lib.gsr_last_error(self.h).decode())
             err.entries = e.value

@@ -116,6 +116,17 @@ int gsr_project(gsr_ctx* ctx, const gsr_camera* cam, const float* gs, const floa
 int gsr_bin_sort(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, uint32_t n, uint32_t* order,
                  uint64_t order_capacity, uint32_t* ranges, uint64_t* num_entries);
 
+/* gsr_bin_sort that allocates nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid it).  Same arguments, results and
+ * errors as gsr_bin_sort.  Its scratch is a 48 MiB static device array per GPU
+ * (held until the process ends), enough for about 1M Gaussians on a 1920x1080
+ * view; a larger input returns GSR_E_CAPACITY before anything is launched.
+ * Calls on one GPU share that array: calls that overlap in time on one GPU
+ * (two contexts, or two threads) race. */
+int gsr_bin_sort_noalloc(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, uint32_t n, uint32_t* order,
+                         uint64_t order_capacity, uint32_t* ranges, uint64_t* num_entries);
+
 /* FORWARD (gpu.js:105-137): front-to-back alpha blend, stop once T <= 1e-4.
  * state may be NULL (the backward recomputes transmittance and needs none). */
 int gsr_forward(gsr_ctx* ctx, const gsr_camera* cam, const float* splats, const uint32_t* order,
