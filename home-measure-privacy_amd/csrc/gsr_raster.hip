@@ -134,12 +134,14 @@ __device__ inline Row row_terms(const Conic2& q, float dx, float rad) {
 
 // alphaAt (gpu.js:92-103) without branches: alpha, or 0 for a miss or for a pixel
 // already done (T <= 1e-4).  The miss tests are the reference's, negated as
-// written there (a NaN takes the same path).  The forward and the backward call
-// this same function, so both see the same alpha bit for bit.
+// written there (a NaN takes the same path).  They are combined with bitwise
+// ors, so the compiler selects instead of masking lanes around the exponential
+// (the masked form cost both blends about 6%).  The forward and the backward
+// call this same function, so both see the same alpha bit for bit.
 __device__ inline float alpha_at(float dy, const Conic2& q, const Row& w, float op, float rad, float T) {
   const float p2 = fmaf(dy, fmaf(q.qC, dy, w.nqbdx), w.qadx2);
   const float alpha = fminf(kAlphaMax, op * __builtin_amdgcn_exp2f(p2));
-  const bool miss = w.missx || fabsf(dy) > rad || p2 > 0.0f || alpha < kAlphaMin || !(T > kTStop);
+  const bool miss = w.missx | (fabsf(dy) > rad) | (p2 > 0.0f) | (alpha < kAlphaMin) | !(T > kTStop);
   return miss ? 0.0f : alpha;
 }
 
@@ -541,11 +543,7 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
       const Conic2 q{r3.x, r3.y, r3.z};
       const float dx = px - u;
       const Row row = row_terms(q, dx, rad);
-      const float omo = 1.0f - op;
-      const float Adx = A * dx, Bdx = B * dx, ndx = -dx, nhdx2 = -0.5f * dx * dx;
-      float g[9];
-#pragma unroll
-      for (int m = 0; m < 9; ++m) g[m] = 0.0f;
+      float gc0 = 0.0f, gc1 = 0.0f, gc2 = 0.0f, s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
       bool hit = false;
 #pragma unroll
       for (int i = 0; i < kPix; ++i) {
@@ -568,9 +566,9 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
         if (__ballot(h) == 0) continue;
         hit = true;
         // gpu.js:266-268: colour gradient and dLoss/dalpha = d . (Tk c - behind / (1 - alpha)).
-        g[0] = fmaf(dr[i], wgt, g[0]);
-        g[1] = fmaf(dg[i], wgt, g[1]);
-        g[2] = fmaf(db[i], wgt, g[2]);
+        gc0 = fmaf(dr[i], wgt, gc0);
+        gc1 = fmaf(dg[i], wgt, gc1);
+        gc2 = fmaf(db[i], wgt, gc2);
         const float dc = fmaf(dr[i], r2.x, fmaf(dg[i], r2.y, db[i] * r2.z));
         const float behind = fmaf(dr[i], fr[i] - ar[i], fmaf(dg[i], fg[i] - ag[i], db[i] * (fb[i] - ab[i])));
         float dA = fmaf(Tk, dc, -(behind * __builtin_amdgcn_rcpf(oma)));
@@ -579,14 +577,17 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
         dA = (h && alpha < kAlphaMax) ? dA : 0.0f;
         const float dP = dA * alpha;
         const float dPdy = dP * dy;
-        g[3] = fmaf(dP, omo, g[3]);
-        g[4] = fmaf(dPdy, B, fmaf(dP, Adx, g[4]));
-        g[5] = fmaf(dPdy, C, fmaf(dP, Bdx, g[5]));
-        g[6] = fmaf(dP, nhdx2, g[6]);
-        g[7] = fmaf(dPdy, ndx, g[7]);
-        g[8] = fmaf(dPdy, -0.5f * dy, g[8]);
+        s0 += dP;
+        s1 += dPdy;
+        s2 = fmaf(dPdy, dy, s2);
       }
       if (!hit) continue;  // wave-uniform: set only for bands with a hit
+      // gpu.js:270-281 summed over the lane's pixels: with S0 = sum dP, S1 = sum
+      // dP dy and S2 = sum dP dy^2 (dP = dLoss/dpower), d opLogit = (1 - op) S0,
+      // du = A dx S0 + B S1, dv = B dx S0 + C S1, dA = -dx^2/2 S0, dB = -dx S1,
+      // dC = -S2/2 (dx is the lane's, shared by its pixels).
+      const float g[9] = {gc0, gc1, gc2, s0 * (1.0f - op), fmaf(s1, B, s0 * (A * dx)), fmaf(s1, C, s0 * (B * dx)),
+                          s0 * (-0.5f * dx * dx), -(s1 * dx), -0.5f * s2};
       const float val = reduce9(g, lane);
       if (slot >= 0 && val != 0.0f) unsafeAtomicAdd(grads + size_t(gi_lds[j]) * record + slot, val);
     }
