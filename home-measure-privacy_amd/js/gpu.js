@@ -10,6 +10,13 @@
 // Optional SH: gaussians[i].sh = 48 floats ([16][3], coefficient 0 = colour).
 // The device-free helpers (packing, .ply / .splat, densify, ...) live in
 // formats.js and are re-exported here.
+//
+// As in the reference, the renderer's functions are async: device work is
+// queued in order, and readbacks (read(), gpu.js:716-726) and the tile-list
+// entry count wait for the device on a worker thread of the addon, so the event
+// loop keeps running meanwhile.  Extension: gradientsBatch() renders many views
+// in one batched step, and trainScene({batchViews}) trains on several views per
+// step that way.
 
 const path = require('path');
 const addon = require(path.join(__dirname, 'gsr.node'));
@@ -60,6 +67,26 @@ function tileLists(splats, n, width, height) {
   }
 }
 
+/** The 80-byte camera uniforms of several views, concatenated (batched calls). */
+function camerasUniform(cams, count) {
+  const out = new Uint8Array(cams.length * 80);
+  cams.forEach((c, v) => out.set(new Uint8Array(cameraUniform(c, count)), v * 80));
+  return out;
+}
+
+/** The views' RGB images (any numeric arrays) divided by `divisor`, concatenated as float32. */
+function concatTargets(images, pixels, divisor) {
+  const total = pixels.reduce((a, b) => a + b, 0);
+  const out = new Float32Array(total * 3);
+  let off = 0;
+  images.forEach((img, v) => {
+    const m = pixels[v] * 3;
+    for (let i = 0; i < m; i++) out[off + i] = img[i] / divisor;
+    off += m;
+  });
+  return out;
+}
+
 function shOf(gaussians) {
   if (!gaussians.length || !gaussians[0].sh) return null;
   const out = new Float32Array(gaussians.length * 48);
@@ -89,10 +116,14 @@ async function createRenderer({ gpuTiles = true, device = 0 } = {}) { // eslint-
     return ptr;
   }
   function pooled(name) { return pool.get(name).bytes; }
+  // read() of gpu.js:716-726: a copy of device memory once the queued work is done.
+  async function readF32(ptr, count) {
+    return new Float32Array(await addon.read(ctx, ptr, count * 4));
+  }
 
   // project -> tile lists -> blend (forward(), gpu.js:783-812).  With `reuse`
   // the buffers come from the pool (training) and stay there.
-  function forward(gBuf, shBuf, shDegree, n, cam, reuse = false) {
+  async function forward(gBuf, shBuf, shDegree, n, cam, reuse = false) {
     const get = reuse ? scratch : (name, bytes) => buffer(bytes);
     const camBuf = cameraUniform(cam, n);
     const sBuf = get('splats', n * 48);
@@ -102,12 +133,12 @@ async function createRenderer({ gpuTiles = true, device = 0 } = {}) { // eslint-
     let oBuf = reuse ? scratch('order', 4) : null;
     let entries;
     try {
-      entries = addon.binSort(ctx, camBuf, sBuf, n, oBuf, reuse ? pooled('order') / 4 : 0, rBuf);
+      entries = await addon.binSortAsync(ctx, camBuf, sBuf, n, oBuf, reuse ? pooled('order') / 4 : 0, rBuf);
     } catch (e) {
       if (e.code !== GSR_E_CAPACITY) throw e;
       entries = e.entries;
       oBuf = get('order', entries * 4);
-      if (entries > 0) addon.binSort(ctx, camBuf, sBuf, n, oBuf, entries, rBuf);
+      if (entries > 0) await addon.binSortAsync(ctx, camBuf, sBuf, n, oBuf, entries, rBuf);
     }
     if (oBuf === null) oBuf = buffer(4);
     const pixels = cam.width * cam.height;
@@ -131,9 +162,8 @@ async function createRenderer({ gpuTiles = true, device = 0 } = {}) { // eslint-
   /** render (gpu.js:815-820): RGB floats, width * height * 3. */
   async function render(gaussians, cam) {
     const u = upload(gaussians);
-    const f = forward(u.gBuf, u.shBuf, u.shDegree, gaussians.length, cam);
-    const out = new Float32Array(f.pixels * 3);
-    addon.download(ctx, f.iBuf, out);
+    const f = await forward(u.gBuf, u.shBuf, u.shDegree, gaussians.length, cam);
+    const out = await readF32(f.iBuf, f.pixels * 3);
     release([u.gBuf, u.shBuf, f.sBuf, f.oBuf, f.rBuf, f.iBuf]);
     return out;
   }
@@ -142,7 +172,7 @@ async function createRenderer({ gpuTiles = true, device = 0 } = {}) { // eslint-
   async function gradients(gaussians, cam, target, ssim = 0) {
     const n = gaussians.length;
     const u = upload(gaussians);
-    const f = forward(u.gBuf, u.shBuf, u.shDegree, n, cam);
+    const f = await forward(u.gBuf, u.shBuf, u.shDegree, n, cam);
     const tBuf = buffer(f.pixels * 12);
     addon.upload(ctx, tBuf, Float32Array.from(target));
     const lg = buffer(f.pixels * 12);
@@ -152,17 +182,11 @@ async function createRenderer({ gpuTiles = true, device = 0 } = {}) { // eslint-
     const shapeBuf = buffer(n * 40);
     const shGrad = u.shBuf ? buffer(n * 192) : null;
     addon.shape(ctx, f.camBuf, u.gBuf, u.shBuf, u.shDegree, n, gradBuf, shapeBuf, shGrad);
-    const image = new Float32Array(f.pixels * 3);
-    const grads = new Float32Array(n * 16);
-    const shape = new Float32Array(n * 10);
-    addon.download(ctx, f.iBuf, image);
-    addon.download(ctx, gradBuf, grads);
-    addon.download(ctx, shapeBuf, shape);
+    const image = await readF32(f.iBuf, f.pixels * 3);
+    const grads = await readF32(gradBuf, n * 16);
+    const shape = await readF32(shapeBuf, n * 10);
     const out = { image, grads, shape };
-    if (shGrad) {
-      out.shGrad = new Float32Array(n * 48);
-      addon.download(ctx, shGrad, out.shGrad);
-    }
+    if (shGrad) out.shGrad = await readF32(shGrad, n * 48);
     release([u.gBuf, u.shBuf, f.sBuf, f.oBuf, f.rBuf, f.iBuf, tBuf, lg, gradBuf, shapeBuf, shGrad]);
     return out;
   }
@@ -177,10 +201,47 @@ async function createRenderer({ gpuTiles = true, device = 0 } = {}) { // eslint-
     addon.upload(ctx, iBuf, Float32Array.from(image));
     addon.upload(ctx, tBuf, Float32Array.from(target));
     addon.lossGrad(ctx, cam, iBuf, tBuf, lambda, out);
-    const g = new Float32Array(pixels * 3);
-    addon.download(ctx, out, g);
+    const g = await readF32(out, pixels * 3);
     release([iBuf, tBuf, out]);
     return g;
+  }
+
+  /**
+   * gradientsBatch (extension): gradients() of many views of any resolution in
+   * one batched step against one shared set of Gaussians.  targets[v] are the
+   * views' RGB floats.  Returns {images[v], grads (16 per Gaussian, colour and
+   * opacity summed over views), shape (10 per Gaussian, summed), shGrad (48 per
+   * Gaussian, with SH), stats (2 per Gaussian: STATS, gpu.js:483-497, added for
+   * every view), entries[v]}.
+   */
+  async function gradientsBatch(gaussians, cams, targets, ssim = 0) {
+    const n = gaussians.length;
+    const u = upload(gaussians);
+    const camsBuf = camerasUniform(cams, n);
+    const pixels = cams.map((c) => c.width * c.height);
+    const total = pixels.reduce((a, b) => a + b, 0);
+    const bt = addon.batchCreate(ctx, camsBuf, n, u.shDegree);
+    const tBuf = buffer(total * 12), iBuf = buffer(total * 12);
+    const gradBuf = buffer(n * 64), shapeBuf = buffer(n * 40), statsBuf = buffer(n * 8);
+    const shGrad = u.shBuf ? buffer(n * 192) : null;
+    try {
+      addon.upload(ctx, tBuf, concatTargets(targets, pixels, 1));
+      addon.batchStep(bt, u.gBuf, u.shBuf, tBuf, ssim, gradBuf, shapeBuf, shGrad, statsBuf, iBuf, false);
+      const all = await readF32(iBuf, total * 3);
+      const images = [];
+      let off = 0;
+      for (const p of pixels) {
+        images.push(all.slice(off, off + p * 3));
+        off += p * 3;
+      }
+      const out = { images, grads: await readF32(gradBuf, n * 16), shape: await readF32(shapeBuf, n * 10),
+                    stats: await readF32(statsBuf, n * 2), entries: Array.from(addon.batchEntries(bt)) };
+      if (shGrad) out.shGrad = await readF32(shGrad, n * 48);
+      return out;
+    } finally {
+      addon.batchDestroy(bt);
+      release([u.gBuf, u.shBuf, tBuf, iBuf, gradBuf, shapeBuf, statsBuf, shGrad]);
+    }
   }
 
   /** train (gpu.js:846-878): fixed view cycle, L1, Adam on the 14 numbers. */
@@ -198,7 +259,7 @@ async function createRenderer({ gpuTiles = true, device = 0 } = {}) { // eslint-
     const stepData = new ArrayBuffer(32);
     for (let t = 1; t <= steps; t++) {
       const view = views[(t - 1) % views.length];
-      const f = forward(gBuf, null, 0, n, view.cam);
+      const f = await forward(gBuf, null, 0, n, view.cam);
       const lg = buffer(f.pixels * 12);
       addon.lossGrad(ctx, f.camBuf, f.iBuf, targets[(t - 1) % views.length], 0, lg);
       const gradBuf = buffer(n * 64);
@@ -208,15 +269,13 @@ async function createRenderer({ gpuTiles = true, device = 0 } = {}) { // eslint-
       new Float32Array(stepData).set([t, 0, lr.color, lr.opacity, lr.mean, lr.scale, lr.rotation, 0]);
       new Uint32Array(stepData)[1] = n;
       addon.adam(ctx, stepData, gBuf, gradBuf, shapeBuf, moments);
-      const image = new Float32Array(f.pixels * 3);
-      addon.download(ctx, f.iBuf, image);
+      const image = await readF32(f.iBuf, f.pixels * 3);
       let loss = 0;
       for (let i = 0; i < image.length; i++) loss += Math.abs(image[i] - view.target[i]);
       losses.push(loss / image.length);
       release([f.sBuf, f.oBuf, f.rBuf, f.iBuf, lg, gradBuf, shapeBuf]);
     }
-    const data = new Float32Array(n * 16);
-    addon.download(ctx, gBuf, data);
+    const data = await readF32(gBuf, n * 16);
     release([gBuf, moments].concat(targets));
     return { gaussians: unpackGaussians(data), losses };
   }
@@ -227,10 +286,13 @@ async function createRenderer({ gpuTiles = true, device = 0 } = {}) { // eslint-
    * densifyUntil]; `onProgress({step, loss, count})` every 50 steps (awaited);
    * `shouldStop()` ends early.  Views are {cam, image} with image the photo's RGB
    * bytes.  `random` (extension) replaces Math.random for the shuffles and splits.
+   * `batchViews` (extension, default 1): views per step, taken in the same
+   * shuffled order and rendered in one batched step; Adam then steps on their
+   * summed gradients and STATS counts every view.
    */
   async function trainScene(start, views, { steps = 3000, densifyEvery = 100, densifyFrom = 300, densifyUntil = 2000,
                                             maxSplats = 400000, ssim = 0, prune = false, opacityReset = false,
-                                            onProgress, shouldStop, lr, random = Math.random } = {}) {
+                                            onProgress, shouldStop, lr, random = Math.random, batchViews = 1 } = {}) {
     lr = Object.assign({ color: 0.025, opacity: 0.05, mean: 0.002, scale: 0.005, rotation: 0.001 }, lr);
     let gaussians = start;
     let n = gaussians.length;
@@ -250,44 +312,83 @@ async function createRenderer({ gpuTiles = true, device = 0 } = {}) { // eslint-
     const stepData = new ArrayBuffer(32);
     let lossSum = 0, lossCount = 0;
     let order = [];
+    const k = Math.max(1, Math.min(views.length, batchViews | 0));
+    // Batched mode: one batch per sequence of view resolutions met, cameras swapped per step.
+    const batches = new Map();
+    const dropBatches = () => {
+      for (const b of batches.values()) addon.batchDestroy(b);
+      batches.clear();
+    };
+    // Mean absolute error of rendered RGB floats against photos' bytes.
+    const l1 = (image, photos) => {
+      let loss = 0, off = 0;
+      for (const photo of photos) {
+        for (let i = 0; i < photo.length; i++) loss += Math.abs(image[off + i] - photo[i] / 255);
+        off += photo.length;
+      }
+      return loss / image.length;
+    };
     try {
       for (let t = 1; t <= steps; t++) {
         if (shouldStop && shouldStop()) break;
-        if (!order.length) order = shuffled(views.length, random);
-        const view = views[order.pop()];
-        const f = forward(gBuf, null, 0, n, view.cam, true);
-        const tBuf = scratch('target', f.pixels * 12);
-        addon.upload(ctx, tBuf, Float32Array.from(view.image, (v) => v / 255));
-        const gradBuf = scratch('grads', n * 64);
-        addon.clear(ctx, gradBuf, n * 64);
-        const lg = scratch('lossGrad', f.pixels * 12);
-        addon.lossGrad(ctx, f.camBuf, f.iBuf, tBuf, ssim, lg);
-        addon.backward(ctx, f.camBuf, f.sBuf, f.oBuf, f.rBuf, f.iBuf, lg, gradBuf);
-        const shapeBuf = scratch('shape', n * 40);
-        addon.shape(ctx, f.camBuf, gBuf, null, 0, n, gradBuf, shapeBuf, null);
+        const chosen = [];
+        while (chosen.length < k) {
+          if (!order.length) order = shuffled(views.length, random);
+          chosen.push(views[order.pop()]);
+        }
         new Float32Array(stepData).set([t, 0, lr.color, lr.opacity, lr.mean, lr.scale, lr.rotation, 0]);
         new Uint32Array(stepData)[1] = n;
-        addon.stats(ctx, stepData, gradBuf, f.sBuf, statsBuf);
-        addon.adam(ctx, stepData, gBuf, gradBuf, shapeBuf, moments);
-        if (t % 25 === 0) {
-          const image = new Float32Array(f.pixels * 3);
-          addon.download(ctx, f.iBuf, image);
-          let loss = 0;
-          for (let i = 0; i < image.length; i++) loss += Math.abs(image[i] - view.image[i] / 255);
-          lossSum += loss / image.length;
-          lossCount++;
+        if (k === 1) {
+          const view = chosen[0];
+          const f = await forward(gBuf, null, 0, n, view.cam, true);
+          const tBuf = scratch('target', f.pixels * 12);
+          addon.upload(ctx, tBuf, Float32Array.from(view.image, (v) => v / 255));
+          const gradBuf = scratch('grads', n * 64);
+          addon.clear(ctx, gradBuf, n * 64);
+          const lg = scratch('lossGrad', f.pixels * 12);
+          addon.lossGrad(ctx, f.camBuf, f.iBuf, tBuf, ssim, lg);
+          addon.backward(ctx, f.camBuf, f.sBuf, f.oBuf, f.rBuf, f.iBuf, lg, gradBuf);
+          const shapeBuf = scratch('shape', n * 40);
+          addon.shape(ctx, f.camBuf, gBuf, null, 0, n, gradBuf, shapeBuf, null);
+          addon.stats(ctx, stepData, gradBuf, f.sBuf, statsBuf);
+          addon.adam(ctx, stepData, gBuf, gradBuf, shapeBuf, moments);
+          if (t % 25 === 0) {
+            lossSum += l1(await readF32(f.iBuf, f.pixels * 3), [view.image]);
+            lossCount++;
+          }
+        } else {
+          const camsBuf = camerasUniform(chosen.map((v) => v.cam), n);
+          const key = chosen.map((v) => v.cam.width + 'x' + v.cam.height).join(',');
+          let bt = batches.get(key);
+          if (!bt) {
+            bt = addon.batchCreate(ctx, camsBuf, n, 0);
+            batches.set(key, bt);
+          } else {
+            addon.batchSetCameras(bt, camsBuf);
+          }
+          const pixels = chosen.map((v) => v.cam.width * v.cam.height);
+          const total = pixels.reduce((a, b) => a + b, 0);
+          const tBuf = scratch('targets', total * 12);
+          addon.upload(ctx, tBuf, concatTargets(chosen.map((v) => v.image), pixels, 255));
+          const gradBuf = scratch('grads', n * 64), shapeBuf = scratch('shape', n * 40);
+          const iBuf = scratch('images', total * 12);
+          addon.batchStep(bt, gBuf, null, tBuf, ssim, gradBuf, shapeBuf, null, statsBuf, iBuf, true);
+          addon.adam(ctx, stepData, gBuf, gradBuf, shapeBuf, moments);
+          if (t % 25 === 0) {
+            lossSum += l1(await readF32(iBuf, total * 3), chosen.map((v) => v.image));
+            lossCount++;
+          }
         }
         if (t >= densifyFrom && t <= densifyUntil && t % densifyEvery === 0) {
           // per-pixel gradients: the 3DGS threshold (NDC units) scaled by half the width
           const gradThreshold = 2e-4 / (0.5 * views[0].cam.width);
-          const data = new Float32Array(n * 16);
-          const stats = new Float32Array(n * 2);
-          addon.download(ctx, gBuf, data);
-          addon.download(ctx, statsBuf, stats);
+          const data = await readF32(gBuf, n * 16);
+          const stats = await readF32(statsBuf, n * 2);
           gaussians = densify(unpackGaussians(data), stats, { gradThreshold, maxCount: maxSplats,
             sizeThreshold: 0.01 * extent, maxSize: prune ? 0.1 * extent : Infinity }, random);
           if (t === resetAt) for (const g of gaussians) g.opacityLogit = Math.min(g.opacityLogit, lowOpacity);
           upload();
+          dropBatches();
         }
         if (t % 50 === 0) {
           if (onProgress) await onProgress({ step: t, loss: lossCount ? lossSum / lossCount : NaN, count: n });
@@ -296,17 +397,16 @@ async function createRenderer({ gpuTiles = true, device = 0 } = {}) { // eslint-
           await new Promise((r) => setTimeout(r));
         }
       }
-      const data = new Float32Array(n * 16);
-      addon.download(ctx, gBuf, data);
-      return unpackGaussians(data);
+      return unpackGaussians(await readF32(gBuf, n * 16));
     } finally {
+      dropBatches();
       release([gBuf, moments, statsBuf]);
       for (const entry of pool.values()) addon.free(ctx, entry.ptr);
       pool.clear();
     }
   }
 
-  return { device: ctx, render, gradients, train, trainScene, lossGradientOf };
+  return { device: ctx, render, gradients, train, trainScene, lossGradientOf, gradientsBatch };
 }
 
 module.exports = {

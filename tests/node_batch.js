@@ -1,0 +1,49 @@
+'use strict';
+// Drives the batched extension of the JavaScript shim: gradientsBatch() on a
+// scene written by test_node_batch.py, and the addon's busy guard while a read()
+// is pending.  Writes the results as raw float32 files.
+const fs = require('fs');
+const path = require('path');
+const gpu = require(path.join(__dirname, '..', 'home-measure-privacy_amd', 'js', 'gpu.js'));
+const addon = require(path.join(__dirname, '..', 'home-measure-privacy_amd', 'js', 'gsr.node'));
+
+function f32(file) {
+  const b = fs.readFileSync(file);
+  return new Float32Array(b.buffer, b.byteOffset, b.byteLength / 4);
+}
+
+async function main() {
+  const dir = process.argv[2];
+  const meta = JSON.parse(fs.readFileSync(path.join(dir, 'scene.json'), 'utf8'));
+  const gaussians = gpu.unpackGaussians(f32(path.join(dir, 'gs.f32')));
+  const sh = f32(path.join(dir, 'sh.f32'));
+  gaussians.forEach((g, i) => { g.sh = sh.slice(i * 48, i * 48 + 48); });
+  const targets = meta.cams.map((c, v) => f32(path.join(dir, `target${v}.f32`)));
+  const r = await gpu.createRenderer();
+  const out = await r.gradientsBatch(gaussians, meta.cams, targets, meta.ssim);
+  const write = (name, arr) => fs.writeFileSync(path.join(dir, name), Buffer.from(arr.buffer, arr.byteOffset, arr.byteLength));
+  out.images.forEach((img, v) => write(`image${v}.f32`, img));
+  write('grads.f32', out.grads);
+  write('shape.f32', out.shape);
+  write('shgrad.f32', out.shGrad);
+  write('stats.f32', out.stats);
+  // While a read() is pending its context refuses other calls.
+  const ctx = addon.create(0);
+  const buf = addon.alloc(ctx, 1 << 20);
+  const pending = addon.read(ctx, buf, 1 << 20);
+  let busy = false;
+  try {
+    addon.alloc(ctx, 16);
+  } catch (e) {
+    busy = /busy/.test(e.message);
+  }
+  const data = await pending;
+  addon.free(ctx, buf);  // accepted again once the read has settled
+  fs.writeFileSync(path.join(dir, 'done.json'), JSON.stringify({ ok: true, busy, readBytes: data.byteLength,
+    entries: out.entries }));
+}
+
+main().catch((e) => {
+  console.error(e.stack || e.message);
+  process.exit(1);
+});

@@ -25,8 +25,10 @@ def test_addon_exports_every_operator():
     assert out.returncode == 0, out.stderr
     names = set(out.stdout.strip().split(","))
     for op in ("project", "binSort", "forward", "lossGrad", "backward", "shape", "adam", "stats", "alloc", "upload",
-               "download"):
-        assert op in names
+               "download", "read", "binSortAsync", "setCulling", "lossGradWindow", "adamSh", "batchCreate",
+               "batchDestroy", "batchSetCameras", "batchStep", "batchEntries", "batchAutotune", "batchTuning",
+               "batchSetTuning", "batchSetPasses", "batchSetSsimWindow", "batchSetCulling"):
+        assert op in names, op
 
 
 @pytest.mark.gpu
