@@ -1,9 +1,11 @@
 // Internal declarations shared by the HIP translation units of libgsr.
 //
-// Device code follows the WGSL of trainer/gpu.js (cited per function).  All HIP
-// sources are built with -ffp-contract=off: the backward pass replays the
-// forward's blend arithmetic bit for bit, so every fused multiply-add is written
-// out explicitly and identically in both.
+// Device code follows the WGSL of trainer/gpu.js (cited per function).  The
+// blends (gsr_raster.hip) are built with -ffp-contract=off: the backward pass
+// replays the forward's blend arithmetic bit for bit, so every fused
+// multiply-add is written out explicitly and identically in both.  So are the
+// sort and the optimizer (reference op order).  The projection and the chain
+// are built with contraction and hardware-accuracy division (see build.py).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -196,7 +198,7 @@ void launch_loss_grad(hipStream_t s, const ViewDesc* views, const ViewDesc* host
 void launch_copy(hipStream_t s, void* dst, const void* src, size_t bytes);
 void launch_zero(hipStream_t s, void* p, size_t bytes);
 
-// Chain rule to the 3D parameters, Adam, stats (gsr_chain.hip).
+// Chain rule to the 3D parameters (gsr_chain.hip); Adam and stats (gsr_optim.hip).
 // grads: per-view 16-float records, view v at grads + v * grads_view_stride.
 // shape / sh_grad are written with the sum over views; grads_out (optional)
 // receives the summed colour (0-2) and opacity-logit (3) gradients.  With
