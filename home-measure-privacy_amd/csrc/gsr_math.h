@@ -247,10 +247,14 @@ constexpr float kNever = -1e30f;  // extent of an entry that cannot hit any pixe
 // the box evaluates to a miss in the blend's own arithmetic too and skipping it
 // changes no result.  Opacities below ~1/255 get kNever;
 // degenerate conics fall back to the square.
-__device__ inline float2 hit_extent(float A, float B, float C, float op, float r) {
+__device__ inline float hit_threshold(float A, float B, float C, float op, float r) {
   const float r1 = r + 1.0f;
   const float slack = 0.01f + 1e-6f * (fabsf(A) + 2.0f * fabsf(B) + fabsf(C)) * r1 * r1;
-  const float t = __logf(255.0f * op) + slack;
+  return __logf(255.0f * op) + slack;
+}
+
+__device__ inline float2 hit_extent(float A, float B, float C, float op, float r) {
+  const float t = hit_threshold(A, B, C, op, r);
   const float bb = B * B;
   const float det = fmaf(A, C, -bb) - fmaf(B, B, -bb);
   if (t != t || !(det > 0.0f) || !(A > 0.0f) || !(C > 0.0f)) return make_float2(r, r);
@@ -259,6 +263,24 @@ __device__ inline float2 hit_extent(float A, float B, float C, float op, float r
   const float ex = fmaf(sqrtf(k * C), 1.0001f, 1e-3f);
   const float ey = fmaf(sqrtf(k * A), 1.0001f, 1e-3f);
   return make_float2(fminf(r, ex), fminf(r, ey));
+}
+
+// Least value of -2 power = A x^2 + 2 B x y + C y^2 (A, C > 0, A C > B^2) over
+// the rectangle [x0, x1] x [y0, y1] of pixel offsets (x0 <= x1, y0 <= y1): 0 when
+// the rectangle holds the centre, else the least of its four edges' minima, each
+// at the edge's stationary point (y = -B x / C on an edge x = const, x = -B y / A
+// on y = const) clamped into the edge.  A rounding error in that point raises
+// the value only to second order in the error.
+__device__ inline float conic_min_rect(float A, float B, float C, float x0, float x1, float y0, float y1) {
+  const float bc = __fdividef(-B, C), ba = __fdividef(-B, A), b2 = 2.0f * B;
+  const float ya = fminf(fmaxf(bc * x0, y0), y1), yb = fminf(fmaxf(bc * x1, y0), y1);
+  const float xa = fminf(fmaxf(ba * y0, x0), x1), xb = fminf(fmaxf(ba * y1, x0), x1);
+  const float qa = fmaf(x0, fmaf(A, x0, b2 * ya), C * ya * ya);
+  const float qb = fmaf(x1, fmaf(A, x1, b2 * yb), C * yb * yb);
+  const float qc = fmaf(y0, fmaf(C, y0, b2 * xa), A * xa * xa);
+  const float qd = fmaf(y1, fmaf(C, y1, b2 * xb), A * xb * xb);
+  const bool inside = x0 <= 0.0f && x1 >= 0.0f && y0 <= 0.0f && y1 >= 0.0f;
+  return inside ? 0.0f : fminf(fminf(qa, qb), fminf(qc, qd));
 }
 
 }  // namespace gsr

@@ -13,8 +13,8 @@
 //
 // The backward walks the same list front to back and recomputes transmittance
 // with exactly the forward's arithmetic, so it stops at the same entry for each
-// pixel.  The colour behind an entry is C_final - (accumulated colour), C_final
-// being the forward's image.  Per entry, the four pixels of a lane are summed in
+// pixel.  The colour behind an entry starts as the forward's image and loses
+// each entry's contribution as the walk passes it.  Per entry, the four pixels of a lane are summed in
 // registers, the nine terms are reduced over the wave together (cross-half lane
 // swaps, then DPP within rows), and nine lanes issue one float atomic each into
 // the 16-float gradient record.
@@ -211,6 +211,34 @@ __device__ inline bool band_culled(float ylo, float v, float ey) {
   return (ylo + 3.0f - v) < -ey || (ylo - v) > ey;
 }
 
+// Clears the bands in which the entry's hit ellipse (-2 power <= 2 t, t from
+// hit_threshold) misses the rectangle of live pixel centres: the box's columns
+// and the band's rows within the box, both clipped to the hit box.  The hit box
+// bounds the ellipse, so it overshoots an elongated, tilted one by a wide margin;
+// this test does not.  Offsets are the blend's own subtractions px - u, py - v,
+// so the rectangle holds the offsets of every live pixel of the band.  The
+// bound gets a margin of twice hit_threshold's rounding term beyond it, for the
+// rounding of conic_min_rect, so a band is cleared only where the blend's own
+// arithmetic misses at every pixel.  Degenerate conics keep their bands.
+__device__ inline uint32_t ellipse_bands(uint32_t bands, const Batch& b, float band0, const Box& box, float2 e) {
+  const float A = b.r0.z, B = b.r0.w, C = b.r1.x, u = b.r0.x, v = b.r0.y, r = b.r1.z;
+  const float t = hit_threshold(A, B, C, b.r1.y, r);
+  const float bb = B * B;
+  const float det = fmaf(A, C, -bb) - fmaf(B, B, -bb);
+  if (bands == 0 || !(t > 0.0f) || !(det > 0.0f) || !(A > 0.0f) || !(C > 0.0f)) return bands;
+  const float r1 = r + 1.0f;
+  const float lim = 2.0f * t + 4e-6f * (fabsf(A) + 2.0f * fabsf(B) + fabsf(C)) * r1 * r1;
+  const float x0 = fmaxf(box.x0 - u, -e.x), x1 = fminf(box.x1 - u, e.x);
+  const float by0 = fmaxf(box.y0 - v, -e.y), by1 = fminf(box.y1 - v, e.y);
+#pragma unroll 1
+  for (int i = 0; i < kPix; ++i) {
+    const float y0 = fmaxf((band0 + 4.0f * i) - v, by0), y1 = fminf((band0 + 4.0f * i + 3.0f) - v, by1);
+    const bool miss = !(y0 <= y1) || conic_min_rect(A, B, C, x0, x1, y0, y1) > lim;
+    bands &= miss ? ~(1u << i) : ~0u;
+  }
+  return bands;
+}
+
 // LDS layout of a staged entry: (u, v, A, B), (C, opacity, radius, ex), (r, g, b,
 // ey) and (qA, qB, qC, bands).  The fourth quad holds what depends only on the
 // entry and the tile, computed once per entry by its own lane rather than by the
@@ -233,6 +261,7 @@ __device__ inline void stage(float4 (*lds)[4], uint32_t lane, const Batch& b, fl
 #pragma unroll
   for (int i = 0; i < kPix; ++i) bands |= band_culled(band0 + 4.0f * i, b.r0.y, e.y) ? 0u : (1u << i);
   if (box_culled(box, b.r0.x, b.r0.y, e.x, e.y)) bands = 0;
+  if (!kNoCull) bands = ellipse_bands(bands, b, band0, box, e);
   lds[lane][3] = make_float4(q.qA, q.qB, q.qC, __uint_as_float(bands));
 }
 
@@ -486,29 +515,32 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
   const uint32_t x = t.x0 + (lane & 15);
   const float px = float(x) + 0.5f;
   const int slot = reduce9_slot(lane);
-  // Per pixel: dL/dC (d), the final colour (f, the forward's image) and the
-  // colour accumulated so far (acc, the forward's own fma sequence, so f - acc
-  // is exactly the colour behind an entry and exactly 0 after the last hit).
+  // Per pixel: dL/dC (d) and the colour behind the entries walked so far (b:
+  // the forward's image less each walked entry's weighted colour).  Keeping b
+  // rather than the image and the accumulated colour holds the backward to 96
+  // VGPRs, five waves per SIMD (the walk is issue-bound, and the fifth wave
+  // gives the SIMD more instructions to pair: 5.35 -> 4.97 ms on the 32-view
+  // 1M-Gaussian step); b differs from image - accumulated by rounding only.
   const float py0 = float(t.y0 + (lane >> 4)) + 0.5f;  // row of pixel i: py0 + 4 i
-  float T[kPix], ar[kPix], ag[kPix], ab[kPix], fr[kPix], fg[kPix], fb[kPix], dr[kPix], dg[kPix], db[kPix];
+  float T[kPix], br[kPix], bg[kPix], bb[kPix], dr[kPix], dg[kPix], db[kPix];
 #pragma unroll
   for (int i = 0; i < kPix; ++i) {
     uint32_t y = t.y0 + (lane >> 4) + 4 * i;
     bool inside = x < t.width && y < t.height && i >= t.first && i <= t.last;
     T[i] = inside ? 1.0f : 0.0f;
-    ar[i] = ag[i] = ab[i] = fr[i] = fg[i] = fb[i] = dr[i] = dg[i] = db[i] = 0.0f;
+    br[i] = bg[i] = bb[i] = dr[i] = dg[i] = db[i] = 0.0f;
     if (inside) {
       size_t pix = t.pixel_base + size_t(y) * t.width + x;
-      fr[i] = a.image[pix * 3];
-      fg[i] = a.image[pix * 3 + 1];
-      fb[i] = a.image[pix * 3 + 2];
+      br[i] = a.image[pix * 3];
+      bg[i] = a.image[pix * 3 + 1];
+      bb[i] = a.image[pix * 3 + 2];
       if (a.l1_target) {
         // The L1 gradient (lossGradient at lambda 0, gpu.js:389-392), as
         // loss_grad_kernel computes it, without the round trip through dl.
         const float n = float(t.width * t.height * 3u);
-        dr[i] = sign_f(fr[i] - a.l1_target[pix * 3]) / n;
-        dg[i] = sign_f(fg[i] - a.l1_target[pix * 3 + 1]) / n;
-        db[i] = sign_f(fb[i] - a.l1_target[pix * 3 + 2]) / n;
+        dr[i] = sign_f(br[i] - a.l1_target[pix * 3]) / n;
+        dg[i] = sign_f(bg[i] - a.l1_target[pix * 3 + 1]) / n;
+        db[i] = sign_f(bb[i] - a.l1_target[pix * 3 + 2]) / n;
       } else {
         dr[i] = a.dl[pix * 3];
         dg[i] = a.dl[pix * 3 + 1];
@@ -550,10 +582,10 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
       for (int i = 0; i < kPix; ++i) {
         if (((bands >> i) & 1u) == 0) continue;
         const float dy = (py0 + 4.0f * i) - v;
-        // Same alpha, transmittance and accumulated colour as forward_tile, bit for bit.
+        // Same alpha and transmittance as forward_tile, bit for bit.
         const float alpha = alpha_at(dy, q, row, op, rad, T[i]);
         const bool h = alpha != 0.0f;
-        // A miss (alpha 0) leaves T and the accumulated colour bit for bit as they
+        // A miss (alpha 0) leaves T and the colour behind bit for bit as they
         // are, so the state is updated unconditionally (this form compiles
         // without copies of the pixel state); the gradient terms only where a
         // pixel of the band was hit.
@@ -561,9 +593,9 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
         const float wgt = alpha * Tk;
         const float oma = 1.0f - alpha;
         T[i] = Tk * oma;
-        ar[i] = fmaf(r2.x, wgt, ar[i]);
-        ag[i] = fmaf(r2.y, wgt, ag[i]);
-        ab[i] = fmaf(r2.z, wgt, ab[i]);
+        br[i] = fmaf(-r2.x, wgt, br[i]);
+        bg[i] = fmaf(-r2.y, wgt, bg[i]);
+        bb[i] = fmaf(-r2.z, wgt, bb[i]);
         if (__ballot(h) == 0) continue;
         hit = true;
         // gpu.js:266-268: colour gradient and dLoss/dalpha = d . (Tk c - behind / (1 - alpha)).
@@ -571,7 +603,7 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
         gc1 = fmaf(dg[i], wgt, gc1);
         gc2 = fmaf(db[i], wgt, gc2);
         const float dc = fmaf(dr[i], r2.x, fmaf(dg[i], r2.y, db[i] * r2.z));
-        const float behind = fmaf(dr[i], fr[i] - ar[i], fmaf(dg[i], fg[i] - ag[i], db[i] * (fb[i] - ab[i])));
+        const float behind = fmaf(dr[i], br[i], fmaf(dg[i], bg[i], db[i] * bb[i]));
         float dA = fmaf(Tk, dc, -(behind * __builtin_amdgcn_rcpf(oma)));
         // gpu.js:270-281: opacity logit and screen (u, v, conic) gradients, only
         // while the 0.99 clamp is not active.
@@ -597,14 +629,14 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
 }
 
 template <bool kNoCull>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void backward_kernel(RasterArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void backward_kernel(RasterArgs a) {
   __shared__ float4 lds[64][4];
   __shared__ uint32_t gi_lds[64];
   backward_tile<kNoCull>(a, blockIdx.x, lds, gi_lds);
 }
 
 template <bool kNoCull>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void backward_persistent_kernel(RasterArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void backward_persistent_kernel(RasterArgs a) {
   __shared__ float4 lds[64][4];
   __shared__ uint32_t gi_lds[64];
   for_each_item(a, [&](uint32_t w) __attribute__((always_inline)) { backward_tile<kNoCull>(a, w, lds, gi_lds); });
