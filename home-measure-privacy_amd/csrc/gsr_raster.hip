@@ -6,10 +6,11 @@
 // LDS (each lane gathers one 48-byte splat record; the next batch is prefetched
 // into registers while the current one is blended), every lane then reads the
 // batch back as broadcasts.  A wave stops as soon as all of its 256 pixels have
-// T <= 1e-4, so saturated tiles retire early.  Staging also computes each
-// entry's hit box (where alpha can reach 1/255); entries whose box misses the
-// tile's active pixels, and 4-row bands it misses, are skipped without changing
-// any result.
+// T <= 1e-4, so saturated tiles retire early.  Staging also tests each entry's
+// hit region (where alpha can reach 1/255: its bounding box, then the ellipse
+// itself) against the tile's live pixels, band by band; the walk visits only
+// the entries with a band left, and only those bands, without changing any
+// result.
 //
 // The backward walks the same list front to back and recomputes transmittance
 // with exactly the forward's arithmetic, so it stops at the same entry for each
@@ -76,26 +77,31 @@ __device__ __forceinline__ void tile_segment(const RasterArgs& a, int p, Tile& t
   t.order = a.order + eb;
 }
 
-// Records of entries [k0, k0 + 64): lane j fetches entry k0 + j.
+// Records of entries [k0, k0 + 64): lane j fetches entry k0 + j.  The loads
+// are unconditional (lanes past the list read its last entry and are flagged
+// out), so the registers they fill need no merge and nothing waits for them
+// until the batch is staged; the order indices are loaded a batch earlier
+// still, so the records' addresses are ready when they are issued.
 struct Batch {
-  float4 r0, r1, r2;  // (u, v, A, B), (C, opacity, radius, depth), (r, g, b, valid)
+  float4 r0, r1, r2;  // (u, v, A, B), (C, opacity, radius, -), (r, g, b, valid)
   uint32_t gi;
+  bool in;            // entry k0 + lane exists
 };
 
-__device__ inline Batch fetch_batch(const Tile& t, uint32_t k0, uint32_t lane) {
+// Order indices of entries [k0, k0 + 64), clamped to the list; k0 < t.end.
+__device__ inline uint32_t fetch_index(const Tile& t, uint32_t k0, uint32_t lane) {
+  return t.order[min(k0 + lane, t.end - 1u)];
+}
+
+__device__ inline Batch fetch_batch(const Tile& t, uint32_t gi, uint32_t k0, uint32_t lane) {
   Batch b;
-  uint32_t k = k0 + lane;
-  if (k < t.end) {
-    b.gi = t.order[k];
-    const float4* r = reinterpret_cast<const float4*>(t.splats + size_t(b.gi) * 12);
-    b.r0 = r[0];
-    b.r1 = r[1];
-    b.r2 = r[2];
-  } else {
-    b.gi = 0;
-    b.r0 = b.r1 = make_float4(0.f, 0.f, 0.f, 0.f);
-    b.r2 = make_float4(0.f, 0.f, 0.f, 0.f);  // valid = 0
-  }
+  b.gi = gi;
+  b.in = k0 + lane < t.end;
+  const float4* r = reinterpret_cast<const float4*>(t.splats + size_t(gi) * 12);
+  b.r0 = r[0];
+  const float3 r1 = *reinterpret_cast<const float3*>(r + 1);  // the depth (w) is not blended
+  b.r1 = make_float4(r1.x, r1.y, r1.z, 0.0f);
+  b.r2 = r[2];
   return b;
 }
 
@@ -104,7 +110,7 @@ __device__ inline Batch fetch_batch(const Tile& t, uint32_t k0, uint32_t lane) {
 // everywhere either way (gpu.js:93).
 template <bool kNoCull>
 __device__ inline float2 entry_extent(const Batch& b) {
-  if (b.r2.w < 0.5f) return make_float2(kNever, kNever);  // invalid splat
+  if (!b.in || b.r2.w < 0.5f) return make_float2(kNever, kNever);  // past the list, or an invalid splat
   if (kNoCull) return make_float2(__builtin_huge_valf(), __builtin_huge_valf());
   return hit_extent(b.r0.z, b.r0.w, b.r1.x, b.r1.y, b.r1.z);
 }
@@ -247,11 +253,15 @@ __device__ inline uint32_t ellipse_bands(uint32_t bands, const Batch& b, float b
 // whose bit is clear holds no pixel the entry can hit.  With `box` (the live
 // pixels when the batch is staged), an entry whose hit box misses it gets no
 // bands: pixels still live later in the batch lie inside that box, so the walk
-// skips such entries with one scalar test.  (Moving these tests out of the
-// walk's serial chain of compares and branches took the forward from 6.4 to 5.7
-// ms and the backward from 6.4 to 6.0 ms on the 32-view 1M-Gaussian step.)
+// skips such entries.  (Moving these tests out of the walk's serial chain of
+// compares and branches took the forward from 6.4 to 5.7 ms and the backward
+// from 6.4 to 6.0 ms on the 32-view 1M-Gaussian step.)  Returns the bands, which
+// the lane keeps: the walk visits only the entries whose bits are set (a ballot)
+// and takes each one's bits from its lane with readlane, so an entry costs no
+// LDS round trip before its record is read, and a culled one costs nothing
+// (forward 5.5 -> 4.2 ms, backward 5.1 -> 4.8 ms).
 template <bool kNoCull>
-__device__ inline void stage(float4 (*lds)[4], uint32_t lane, const Batch& b, float band0, const Box& box) {
+__device__ inline uint32_t stage(float4 (*lds)[4], uint32_t lane, const Batch& b, float band0, const Box& box) {
   const float2 e = entry_extent<kNoCull>(b);
   lds[lane][0] = b.r0;
   lds[lane][1] = make_float4(b.r1.x, b.r1.y, b.r1.z, e.x);
@@ -263,17 +273,12 @@ __device__ inline void stage(float4 (*lds)[4], uint32_t lane, const Batch& b, fl
   if (box_culled(box, b.r0.x, b.r0.y, e.x, e.y)) bands = 0;
   if (!kNoCull) bands = ellipse_bands(bands, b, band0, box, e);
   lds[lane][3] = make_float4(q.qA, q.qB, q.qC, __uint_as_float(bands));
+  return bands;
 }
 
 // Bands [first, last] of a work item as a bit mask.
 __device__ inline uint32_t item_bands(const Tile& t) {
   return ((1u << (t.last + 1)) - 1u) & ~((1u << t.first) - 1u);
-}
-
-// The staged band bits of entry j (backward staging), as a scalar, restricted to
-// the item's bands.
-__device__ inline uint32_t entry_bands(const float4 (*lds)[4], uint32_t j, uint32_t item) {
-  return __builtin_amdgcn_readfirstlane(__float_as_uint(lds[j][3].w)) & item;
 }
 
 // ---- forward -------------------------------------------------------------------
@@ -313,8 +318,13 @@ __device__ __forceinline__ void forward_tile(const RasterArgs& a, uint32_t w, fl
   }
   const float band0 = float(t.y0) + 0.5f;
   const uint32_t items = item_bands(t);
-  Batch nb = fetch_batch(t, t.start, lane);
-  uint32_t walked = t.start;  // exact, kLast only
+  Batch nb{};
+  uint32_t ngi = 0;  // order indices of the batch after the next
+  if (t.start < t.end) {
+    nb = fetch_batch(t, fetch_index(t, t.start, lane), t.start, lane);
+    ngi = fetch_index(t, t.start + 64, lane);
+  }
+  uint32_t walked = t.start;  // kLast only: up to the entry at which the walk stopped
   uint32_t stop = t.end;      // start of the batch at which the tile had no active pixel
   for (uint32_t k0 = t.start; k0 < t.end; k0 += 64) {
     if (!any_active(T)) {
@@ -323,17 +333,23 @@ __device__ __forceinline__ void forward_tile(const RasterArgs& a, uint32_t w, fl
     }
     const Box sbox = active_box(t.x0, t.y0, T);
     __syncthreads();
-    stage<kNoCull>(lds, lane, nb, band0, sbox);
+    const uint32_t sb = stage<kNoCull>(lds, lane, nb, band0, sbox) & items;
     __syncthreads();
-    if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
-    const uint32_t nk = min(64u, t.end - k0);
-    for (uint32_t j = 0; j < nk; ++j) {
-      if ((j & 15) == 0 && j != 0) {
-        if (!any_active(T)) break;
+    // Unconditional: past the end the loads read the list's last entry and the
+    // batch is flagged out (and never staged).
+    nb = fetch_batch(t, ngi, k0 + 64, lane);
+    ngi = fetch_index(t, k0 + 128, lane);
+    uint64_t live = __ballot(sb != 0u);
+    uint32_t seen = 0;
+    if (kLast) walked = min(k0 + 64u, t.end);
+    while (live != 0) {
+      const uint32_t j = uint32_t(__builtin_ctzll(live));
+      live &= live - 1;
+      if ((++seen & 15u) == 0u && !any_active(T)) {
+        if (kLast) walked = k0 + j;
+        break;
       }
-      if (kLast) walked = k0 + j + 1;
-      const uint32_t bands = entry_bands(lds, j, items);
-      if (bands == 0) continue;
+      const uint32_t bands = __builtin_amdgcn_readlane(sb, j);
       const float4 r0 = lds[j][0], r1 = lds[j][1], r2 = lds[j][2], r3 = lds[j][3];
       const float u = r0.x, v = r0.y, op = r1.y, rad = r1.z;
       const Conic2 q{r3.x, r3.y, r3.z};
@@ -555,22 +571,30 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
   // The passes of a tile follow each other in depth order; pixel state carries over.
   for (int p = 0; p < a.num_passes; ++p) {
   tile_segment(a, p, t);
-  Batch nb = fetch_batch(t, t.start, lane);
+  Batch nb{};
+  uint32_t ngi = 0;  // order indices of the batch after the next
+  if (t.start < t.end) {
+    nb = fetch_batch(t, fetch_index(t, t.start, lane), t.start, lane);
+    ngi = fetch_index(t, t.start + 64, lane);
+  }
   for (uint32_t k0 = t.start; k0 < t.end; k0 += 64) {
     if (!any_active(T)) break;
     const Box sbox = active_box(t.x0, t.y0, T);
     __syncthreads();
-    stage<kNoCull>(lds, lane, nb, band0, sbox);
+    const uint32_t sb = stage<kNoCull>(lds, lane, nb, band0, sbox) & items;
     gi_lds[lane] = nb.gi;
     __syncthreads();
-    if (k0 + 64 < t.end) nb = fetch_batch(t, k0 + 64, lane);
-    const uint32_t nk = min(64u, t.end - k0);
-    for (uint32_t j = 0; j < nk; ++j) {
-      if ((j & 15) == 0 && j != 0) {
-        if (!any_active(T)) break;
-      }
-      const uint32_t bands = entry_bands(lds, j, items);
-      if (bands == 0) continue;
+    // Unconditional: past the end the loads read the list's last entry and the
+    // batch is flagged out (and never staged).
+    nb = fetch_batch(t, ngi, k0 + 64, lane);
+    ngi = fetch_index(t, k0 + 128, lane);
+    uint64_t live = __ballot(sb != 0u);
+    uint32_t seen = 0;
+    while (live != 0) {
+      const uint32_t j = uint32_t(__builtin_ctzll(live));
+      live &= live - 1;
+      if ((++seen & 15u) == 0u && !any_active(T)) break;
+      const uint32_t bands = __builtin_amdgcn_readlane(sb, j);
       const float4 r0 = lds[j][0], r1 = lds[j][1], r2 = lds[j][2], r3 = lds[j][3];
       const float u = r0.x, v = r0.y, A = r0.z, B = r0.w, C = r1.x, op = r1.y, rad = r1.z;
       const Conic2 q{r3.x, r3.y, r3.z};
