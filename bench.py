@@ -28,6 +28,22 @@ from gsr import lib, scenes  # noqa: E402
 
 METRIC = "views/s fwd+bwd, 1M Gaussians: 32-view batched vs 1-view plain; % HBM peak"
 HBM_PEAK_GBS = 8000.0
+# The tuning knobs (tile_batch, chunk, split_first, split_later, split_backward,
+# emit_band) each stage's kernels read: a stage's committed counter profile holds
+# for every tuning that agrees on these.  The forward splits its heavy tiles by
+# split_first / split_later, the backward by split_backward; binning reads chunk
+# and emit_band; projection and the chain read none.
+STAGE_KNOBS = {"forward": (0, 2, 3), "backward": (0, 4), "bin_sort": (1, 5), "project": (), "chain": ()}
+PROFILE = os.path.join(ROOT, "profiles", "traffic_batched_1M_32views.json")
+
+
+def stage_profile(stage, tuning):
+    """The committed counter profile (HBM traffic, VALU instructions per step,
+    dual-issued fraction) of `stage` at the knobs of `tuning` it reads, or None."""
+    if not os.path.exists(PROFILE):
+        return None
+    key = ",".join(str(tuning[i]) for i in STAGE_KNOBS[stage])
+    return json.load(open(PROFILE)).get("by_knobs", {}).get(stage, {}).get(key)
 SH_BYTES = 192
 
 
@@ -233,9 +249,32 @@ def copy_calibration(mib: int):
     ctx.close()
 
 
+def measure_valu_peak(ctx):
+    """Sustained dual-issue VALU rate of this GPU (wave64 instructions/s): the
+    calibration kernel's fma chains on VGPR operands, timed with HIP events.
+    None when the calibration library is not built."""
+    import ctypes
+
+    path = os.path.join(ROOT, "home-measure-privacy_amd", "gsr", "libgsr_calib.so")
+    if not os.path.exists(path):
+        return None
+    cal = ctypes.CDLL(path)
+    cus, khz = ctypes.c_int(0), ctypes.c_int(0)
+    if cal.gsr_calib_device(ctypes.byref(cus), ctypes.byref(khz)) != 0:
+        return None
+    blocks, trips = cus.value * 8, 8192
+    out = ctx.alloc(blocks * 256 * 4)
+    ms, fma = ctypes.c_float(0), ctypes.c_uint64(0)
+    rc = cal.gsr_calib_valu(ctypes.c_void_p(out.ptr), ctypes.c_uint32(blocks), ctypes.c_uint32(trips),
+                            ctypes.byref(ms), ctypes.byref(fma))
+    out.free()
+    return None if rc != 0 else blocks * 4 * fma.value / (ms.value * 1e-3)
+
+
 def valu_peak():
     """The sustained VALU issue rate of the chip: independent non-packed fma chains
-    (csrc/calib) on 8 blocks of 4 waves per CU.  One JSON line."""
+    on VGPR operands (csrc/calib) on 8 blocks of 4 waves per CU, and the issue rate
+    of each instruction form the blend kernels use.  One JSON line."""
     import ctypes
 
     ctx = lib.Context(0)
@@ -250,13 +289,24 @@ def valu_peak():
     assert rc == 0, rc
     waves = blocks * 4
     instr = waves * fma.value  # wave-level v_fma_f32 instructions
-    rate = instr / (ms.value * 1e-3)
+    peak_ms = ms.value
+    rate = instr / (peak_ms * 1e-3)
     simd_cycles = cus.value * 4 * khz.value * 1e3
+    cal.gsr_calib_issue_name.restype = ctypes.c_char_p
+    forms = []
+    for k in range(cal.gsr_calib_issue_forms()):
+        per = ctypes.c_uint64(0)
+        rc = cal.gsr_calib_issue(k, ctypes.c_void_p(out.ptr), ctypes.c_uint32(blocks), ctypes.c_uint32(trips // 4),
+                                 ctypes.byref(ms), ctypes.byref(per))
+        assert rc == 0, rc
+        r = waves * per.value / (ms.value * 1e-3)
+        forms.append({"form": cal.gsr_calib_issue_name(k).decode(), "wave_instructions_per_s": r,
+                      "per_simd_per_cycle": round(r / simd_cycles, 4)})
     print(json.dumps({"kernel": "valu_peak_kernel", "cus": cus.value, "peak_clock_mhz": khz.value / 1e3,
-                      "waves": waves, "valu_wave_instructions": instr, "ms": round(ms.value, 4),
+                      "waves": waves, "valu_wave_instructions": instr, "ms": round(peak_ms, 4),
                       "valu_wave_instructions_per_s": rate,
                       "per_simd_per_cycle_at_peak_clock": round(rate / simd_cycles, 4),
-                      "fp32_tflops_nonpacked": round(rate * 64 * 2 / 1e12, 2)}), flush=True)
+                      "fp32_tflops_nonpacked": round(rate * 64 * 2 / 1e12, 2), "issue_forms": forms}), flush=True)
     out.free()
     ctx.close()
 
@@ -423,17 +473,32 @@ def main():
     bwd_achieved = bytes_["backward"] / (timed["backward"] * 1e-3) / 1e9
     # HBM bytes per step of the dominant stage's kernel(s), from the rocprofv3
     # FETCH_SIZE / WRITE_SIZE passes committed for this workload (copy-calibrated),
-    # profiled per tuning: reported only for the knobs this run timed.
+    # reported only when the profile covers the knobs the stage ran with here.
     traffic, traffic_note = None, None
-    tpath = os.path.join(ROOT, "profiles", "traffic_batched_1M_32views.json")
-    key = ",".join(str(x) for x in tuning)
-    if (n, V) == (1_000_000, 32) and os.path.exists(tpath):
-        by_tuning = json.load(open(tpath)).get("by_tuning", {})
-        if key in by_tuning:
-            t = by_tuning[key]["stages"].get(dominant)
-            traffic = None if t is None else round(t)
+    full = (n, V) == (1_000_000, 32)
+    if full:
+        prof = stage_profile(dominant, tuning)
+        if prof is None:
+            traffic_note = f"no FETCH_SIZE / WRITE_SIZE profile of {dominant} at these knobs"
         else:
-            traffic_note = f"no FETCH_SIZE / WRITE_SIZE profile for tuning {key} (profiled: {sorted(by_tuning)})"
+            traffic = round(prof["traffic"])
+
+    # Compute roofline of the slower blend kernel: its VALU instructions per step
+    # (rocprofv3 SQ_INSTS_VALU, committed beside the traffic) over its HIP-event
+    # time, against the chip's dual-issue VALU rate measured here.
+    compute, compute_note = None, None
+    blend = max(("forward", "backward"), key=lambda k: timed[k])
+    if full:
+        prof = stage_profile(blend, tuning)
+        if prof is None:
+            compute_note = f"no SQ_INSTS_VALU profile of {blend} at these knobs"
+        else:
+            vpeak = measure_valu_peak(ctx)
+            rate = prof["valu"] / (timed[blend] * 1e-3)
+            compute = {"bound": "valu-issue", "kernel": blend, "unit": "wave64 VALU instructions/s",
+                       "achieved": round(rate, -6), "peak": None if vpeak is None else round(vpeak, -6),
+                       "frac": None if vpeak is None else round(rate / vpeak, 4),
+                       "valu_instructions_per_step": prof["valu"], "dual_issued_frac": prof.get("dual")}
 
     plain = batched_v1 = None
     if not args.no_plain:
@@ -478,7 +543,8 @@ def main():
         },
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     **({"traffic_note": traffic_note} if traffic_note else {})},
+                     **({"traffic_note": traffic_note} if traffic_note else {}),
+                     "compute": compute, **({"compute_note": compute_note} if compute_note else {})},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
