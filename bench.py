@@ -311,6 +311,29 @@ def valu_peak():
     ctx.close()
 
 
+def write_combine():
+    """The write-combining probe (csrc/calib): 256 MB written in full four ways:
+    coalesced; each sector by eight stores of one wave; by eight workgroups on
+    eight dies; by eight workgroups on one die.  Run it under rocprofv3 --pmc
+    WRITE_SIZE to see which partial stores the L2 merges.  One JSON line."""
+    import ctypes
+
+    ctx = lib.Context(0)
+    cal = ctypes.CDLL(os.path.join(ROOT, "home-measure-privacy_amd", "gsr", "libgsr_calib.so"))
+    n = 64 << 20
+    buf = ctx.alloc(n * 4)
+    names = ["coalesced", "one_wave_eight_stores", "eight_workgroups_eight_dies", "eight_workgroups_one_die"]
+    out = {}
+    for mode, name in enumerate(names):
+        ms = ctypes.c_float(0)
+        rc = cal.gsr_calib_wcomb(mode, ctypes.c_void_p(buf.ptr), ctypes.c_uint32(n), ctypes.byref(ms))
+        assert rc == 0, rc
+        out[name] = {"ms": round(ms.value, 4), "useful_GBps": round(n * 4 / (ms.value * 1e-3) / 1e9, 1)}
+    print(json.dumps({"probe": "write_combine", "bytes": n * 4, "kernels": out}), flush=True)
+    buf.free()
+    ctx.close()
+
+
 def tune(args, batch, gsb, shb, tgt, grads, shape, shg):
     """Sets the batch's knobs from --tune, or times candidates (autotune)."""
     if args.tune:
@@ -414,6 +437,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0, help="only time a device copy of this many MiB")
     ap.add_argument("--valu-peak", action="store_true", help="only measure the sustained VALU issue rate")
+    ap.add_argument("--write-combine", action="store_true", help="only run the L2 write-combining probe")
     ap.add_argument("--train", action="store_true", help="the 3M-Gaussian, 64-view training loop instead")
     ap.add_argument("--train-steps", type=int, default=200)
     args = ap.parse_args()
@@ -423,6 +447,9 @@ def main():
         sys.exit(2)
     if args.copy_mib:
         copy_calibration(args.copy_mib)
+        return
+    if args.write_combine:
+        write_combine()
         return
     if args.valu_peak:
         valu_peak()

@@ -95,6 +95,41 @@ constexpr auto issue_table(std::integer_sequence<int, Ks...>) {
   return std::array<IssueKernel, sizeof...(Ks)>{issue_kernel<Ks>...};
 }
 
+// Write-combining probe: the same buffer of n floats (n a multiple of 16384)
+// written in full four ways, to see when the L2 merges partial stores into one
+// memory write (rocprofv3 WRITE_SIZE per kernel).  A sector is 8 floats.
+//   0: coalesced, thread i writes float i;
+//   1: each lane fills its own sector with 8 separate stores (8 instructions);
+//   2: word k of every sector of a group is written by block 8 g + k: the eight
+//      writers of a sector are consecutive workgroups, on eight different dies
+//      (workgroup L runs on die L mod 8);
+//   3: as 2, but the eight writers are workgroups L, L + 8, ..., L + 56 of a
+//      group of 64, all on die L mod 8.
+__global__ __launch_bounds__(256) void wcomb_full(float* __restrict__ b, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) b[i] = float(i);
+}
+__global__ __launch_bounds__(256) void wcomb_same_wave(float* __restrict__ b, uint32_t n) {
+  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= n / 8) return;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    b[size_t(s) * 8 + k] = float(k);
+    __builtin_amdgcn_s_sleep(1);  // separate instructions, apart in time
+  }
+}
+__global__ __launch_bounds__(256) void wcomb_cross_die(float* __restrict__ b, uint32_t n) {
+  const uint32_t k = blockIdx.x % 8, g = blockIdx.x / 8;
+  const uint32_t s = g * 256 + threadIdx.x;
+  if (s < n / 8) b[size_t(s) * 8 + k] = float(k);
+}
+__global__ __launch_bounds__(256) void wcomb_same_die(float* __restrict__ b, uint32_t n) {
+  const uint32_t L = blockIdx.x, r = L % 64;
+  const uint32_t g = (L / 64) * 8 + r % 8, k = r / 8;  // group of die r % 8, word r / 8
+  const uint32_t s = g * 256 + threadIdx.x;
+  if (s < n / 8) b[size_t(s) * 8 + k] = float(k);
+}
+
 // Times `launch` (a warm-up launch first) with HIP events on the default stream.
 template <class F>
 hipError_t time_launch(F&& launch, float* ms) {
@@ -157,6 +192,21 @@ int gsr_calib_issue(int k, float* out, uint32_t blocks, uint32_t trips, float* m
   const hipError_t e =
       time_launch([&] { hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, 0, out, trips, 0.999f); }, ms);
   *instr_per_thread = uint64_t(trips) * kChains * kUnroll * kIssuePer[k];
+  return int(e);
+}
+
+// Runs write-combining probe `mode` (0-3, see above) on b[0, n), n a multiple
+// of 16384 floats; *ms receives the kernel time.  Returns 0 or the HIP error.
+int gsr_calib_wcomb(int mode, float* b, uint32_t n, float* ms) {
+  if (mode < 0 || mode > 3 || n == 0 || n % 16384 != 0) return int(hipErrorInvalidValue);
+  const uint32_t sectors = n / 8;
+  hipError_t e;
+  switch (mode) {
+    case 0: e = time_launch([&] { hipLaunchKernelGGL(wcomb_full, dim3(n / 256), dim3(256), 0, 0, b, n); }, ms); break;
+    case 1: e = time_launch([&] { hipLaunchKernelGGL(wcomb_same_wave, dim3(sectors / 256), dim3(256), 0, 0, b, n); }, ms); break;
+    case 2: e = time_launch([&] { hipLaunchKernelGGL(wcomb_cross_die, dim3(sectors / 256 * 8), dim3(256), 0, 0, b, n); }, ms); break;
+    default: e = time_launch([&] { hipLaunchKernelGGL(wcomb_same_die, dim3(sectors / 256 * 8), dim3(256), 0, 0, b, n); }, ms); break;
+  }
   return int(e);
 }
 
