@@ -14,8 +14,9 @@
 //
 // The backward walks the same list front to back and recomputes transmittance
 // with exactly the forward's arithmetic, so it stops at the same entry for each
-// pixel.  The colour behind an entry starts as the forward's image and loses
-// each entry's contribution as the walk passes it.  Per entry, the four pixels of a lane are summed in
+// pixel.  The colour behind an entry enters only as d . behind (d = dL/dC): it
+// starts as d . image and loses d . c times each entry's weight as the walk
+// passes it.  Per entry, the four pixels of a lane are summed in
 // registers, the nine terms are reduced over the wave together (cross-half lane
 // swaps, then DPP within rows), and nine lanes issue one float atomic each into
 // the 16-float gradient record.
@@ -531,37 +532,38 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
   const uint32_t x = t.x0 + (lane & 15);
   const float px = float(x) + 0.5f;
   const int slot = reduce9_slot(lane);
-  // Per pixel: dL/dC (d) and the colour behind the entries walked so far (b:
-  // the forward's image less each walked entry's weighted colour).  Keeping b
-  // rather than the image and the accumulated colour holds the backward to 96
-  // VGPRs, five waves per SIMD (the walk is issue-bound, and the fifth wave
-  // gives the SIMD more instructions to pair: 5.35 -> 4.97 ms on the 32-view
-  // 1M-Gaussian step); b differs from image - accumulated by rounding only.
+  // Per pixel: dL/dC (d) and bd = d . (the colour behind the entries walked so
+  // far): d . image less d . c times each walked entry's weight.  The behind
+  // term of gpu.js:266-268 needs the behind colour only dotted with d, and d . c
+  // is needed for the gradient anyway, so one running scalar replaces the
+  // image and the accumulated colour (6 values per pixel): five waves per SIMD
+  // instead of four, and four fewer instructions per evaluated band (backward
+  // 5.35 -> 4.46 ms on the 32-view 1M-Gaussian step over the two changes).  bd
+  // differs from d . (image - accumulated) by rounding only.
   const float py0 = float(t.y0 + (lane >> 4)) + 0.5f;  // row of pixel i: py0 + 4 i
-  float T[kPix], br[kPix], bg[kPix], bb[kPix], dr[kPix], dg[kPix], db[kPix];
+  float T[kPix], bd[kPix], dr[kPix], dg[kPix], db[kPix];
 #pragma unroll
   for (int i = 0; i < kPix; ++i) {
     uint32_t y = t.y0 + (lane >> 4) + 4 * i;
     bool inside = x < t.width && y < t.height && i >= t.first && i <= t.last;
     T[i] = inside ? 1.0f : 0.0f;
-    br[i] = bg[i] = bb[i] = dr[i] = dg[i] = db[i] = 0.0f;
+    bd[i] = dr[i] = dg[i] = db[i] = 0.0f;
     if (inside) {
       size_t pix = t.pixel_base + size_t(y) * t.width + x;
-      br[i] = a.image[pix * 3];
-      bg[i] = a.image[pix * 3 + 1];
-      bb[i] = a.image[pix * 3 + 2];
+      const float ir = a.image[pix * 3], ig = a.image[pix * 3 + 1], ib = a.image[pix * 3 + 2];
       if (a.l1_target) {
         // The L1 gradient (lossGradient at lambda 0, gpu.js:389-392), as
         // loss_grad_kernel computes it, without the round trip through dl.
         const float n = float(t.width * t.height * 3u);
-        dr[i] = sign_f(br[i] - a.l1_target[pix * 3]) / n;
-        dg[i] = sign_f(bg[i] - a.l1_target[pix * 3 + 1]) / n;
-        db[i] = sign_f(bb[i] - a.l1_target[pix * 3 + 2]) / n;
+        dr[i] = sign_f(ir - a.l1_target[pix * 3]) / n;
+        dg[i] = sign_f(ig - a.l1_target[pix * 3 + 1]) / n;
+        db[i] = sign_f(ib - a.l1_target[pix * 3 + 2]) / n;
       } else {
         dr[i] = a.dl[pix * 3];
         dg[i] = a.dl[pix * 3 + 1];
         db[i] = a.dl[pix * 3 + 2];
       }
+      bd[i] = fmaf(dr[i], ir, fmaf(dg[i], ig, db[i] * ib));
     }
   }
   const float band0 = float(t.y0) + 0.5f;
@@ -609,26 +611,23 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
         // Same alpha and transmittance as forward_tile, bit for bit.
         const float alpha = alpha_at(dy, q, row, op, rad, T[i]);
         const bool h = alpha != 0.0f;
-        // A miss (alpha 0) leaves T and the colour behind bit for bit as they
-        // are, so the state is updated unconditionally (this form compiles
+        // A miss (alpha 0) leaves T and bd bit for bit as they are, so the
+        // state is updated unconditionally (this form compiles
         // without copies of the pixel state); the gradient terms only where a
         // pixel of the band was hit.
         const float Tk = T[i];
         const float wgt = alpha * Tk;
         const float oma = 1.0f - alpha;
         T[i] = Tk * oma;
-        br[i] = fmaf(-r2.x, wgt, br[i]);
-        bg[i] = fmaf(-r2.y, wgt, bg[i]);
-        bb[i] = fmaf(-r2.z, wgt, bb[i]);
+        const float dc = fmaf(dr[i], r2.x, fmaf(dg[i], r2.y, db[i] * r2.z));
+        bd[i] = fmaf(-dc, wgt, bd[i]);
         if (__ballot(h) == 0) continue;
         hit = true;
         // gpu.js:266-268: colour gradient and dLoss/dalpha = d . (Tk c - behind / (1 - alpha)).
         gc0 = fmaf(dr[i], wgt, gc0);
         gc1 = fmaf(dg[i], wgt, gc1);
         gc2 = fmaf(db[i], wgt, gc2);
-        const float dc = fmaf(dr[i], r2.x, fmaf(dg[i], r2.y, db[i] * r2.z));
-        const float behind = fmaf(dr[i], br[i], fmaf(dg[i], bg[i], db[i] * bb[i]));
-        float dA = fmaf(Tk, dc, -(behind * __builtin_amdgcn_rcpf(oma)));
+        float dA = fmaf(Tk, dc, -(bd[i] * __builtin_amdgcn_rcpf(oma)));
         // gpu.js:270-281: opacity logit and screen (u, v, conic) gradients, only
         // while the 0.99 clamp is not active.
         dA = (h && alpha < kAlphaMax) ? dA : 0.0f;
