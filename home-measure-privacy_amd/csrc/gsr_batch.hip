@@ -30,7 +30,8 @@ int ctx_fail(gsr_ctx* ctx, int code, const char* msg);
 hipError_t ctx_malloc(gsr_ctx* ctx, void** p, size_t bytes);
 }  // namespace gsr
 
-constexpr uint32_t kQueueBase = 32;
+constexpr uint32_t kLiveBase = 32;
+constexpr uint32_t kQueueBase = 48;
 // Per-view gradient records hold the nine screen-space terms at 0-8, packed 12
 // floats apart (48 bytes: wave reads are contiguous, no padding fetched).
 constexpr uint32_t kRecord = 12;
@@ -72,6 +73,8 @@ struct gsr_batch {
   uint32_t* walk = nullptr;     // [kMaxPasses][total_tiles]: entries the forward examined per pass
   uint32_t* prev_walk = nullptr;  // [total_tiles]: the previous step's first-slice walk (a cost estimate)
   TileWork* heavy = nullptr;    // [2 total_tiles]: half-tile items of the split tiles of a launch
+  TileWork* live_work = nullptr;  // [total_tiles]: the whole-tile items of a launch (live list)
+  uint32_t* live_counts = nullptr;  // [ceil(total_tiles / 256)]: live list scratch
   uint8_t* split = nullptr;     // [total_tiles]: tile split in the current launch
   // Tiles are blended as two half-tile items when their cost exceeds these (0: never):
   // the forward of the first depth slice (most tiles saturate early, so list
@@ -84,8 +87,9 @@ struct gsr_batch {
   uint32_t* order = nullptr;
   uint64_t order_cap = 0;
   // [0] overflow, [16 + p] split-item count of forward pass p, [16 + kMaxPasses]
-  // the backward's; from kQueueBase, 9 work counters per forward pass, then the
-  // backward's (a shared one and one per die part).
+  // the backward's; [kLiveBase + p] live-list length of forward pass p,
+  // [kLiveBase + kMaxPasses] the backward's; from kQueueBase, 9 work counters
+  // per forward pass, then the backward's (a shared one and one per die part).
   uint32_t* flags = nullptr;
   float* image = nullptr;
   float* dl = nullptr;
@@ -172,6 +176,8 @@ int alloc_all(gsr_batch* b) {
       (rc = dalloc(b, &b->walk, size_t(kMaxPasses) * b->total_tiles)) ||
       (rc = dalloc(b, &b->prev_walk, b->total_tiles)) ||
       (rc = dalloc(b, &b->heavy, size_t(b->total_tiles) * 2)) || (rc = dalloc(b, &b->split, b->total_tiles)) ||
+      (rc = dalloc(b, &b->live_work, b->total_tiles)) ||
+      (rc = dalloc(b, &b->live_counts, (size_t(b->total_tiles) + 255) / 256 + 1)) ||
       (rc = dalloc(b, &b->flags, kFlagWords)) || (rc = dalloc(b, &b->image, b->total_pixels * 3)) ||
       (rc = dalloc(b, &b->dl, b->total_pixels * 3)) ||
       (rc = dalloc(b, &b->vgrads, size_t(b->V) * std::max<uint32_t>(b->n, 1) * kRecord)) ||
@@ -312,6 +318,27 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
       sa.heavy_count = b->flags + 16 + p;
       launch_split_list(s, sa);
     }
+    // Later slices blend only the live tiles: most are saturated by then, and a
+    // saturated or empty tile writes nothing a later stage reads.  (The first
+    // slice blends every tile: an empty one still writes its image, T and
+    // open bands.)
+    a.work = b->d_work;
+    a.work_count = nullptr;
+    if (p > 0) {
+      LiveArgs la{};
+      la.tiles = b->d_work;
+      la.total_tiles = b->total_tiles;
+      la.ranges[0] = a.ranges[p];
+      la.num_passes = 1;
+      la.done = b->done;
+      la.split = a.split;
+      la.counts = b->live_counts;
+      la.out = b->live_work;
+      la.out_count = b->flags + kLiveBase + p;
+      launch_live_list(s, la);
+      a.work = b->live_work;
+      a.work_count = la.out_count;
+    }
     a.walk = b->walk + size_t(p) * b->total_tiles;
     a.last = prof && prof->last ? prof->last + size_t(p) * b->total_tiles : nullptr;
     launch_forward(s, a, 1, b->grid_fwd);
@@ -356,6 +383,22 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
     sa.heavy = b->heavy;
     sa.heavy_count = b->flags + 16 + kMaxPasses;
     launch_split_list(s, sa);
+  }
+  // The backward blends the tiles with entries in some slice (it writes only
+  // gradients, so an empty tile has nothing to do).
+  {
+    LiveArgs la{};
+    la.tiles = b->d_work;
+    la.total_tiles = b->total_tiles;
+    for (uint32_t p = 0; p < b->num_passes; ++p) la.ranges[p] = a.ranges[p];
+    la.num_passes = int(b->num_passes);
+    la.split = a.split;
+    la.counts = b->live_counts;
+    la.out = b->live_work;
+    la.out_count = b->flags + kLiveBase + kMaxPasses;
+    launch_live_list(s, la);
+    a.work = b->live_work;
+    a.work_count = la.out_count;
   }
   a.queue = b->flags + kQueueBase + 9 * kMaxPasses;
   launch_backward(s, a, 1, b->grid_bwd);
@@ -476,7 +519,7 @@ void gsr_batch_destroy(gsr_batch* b) {
   void* ptrs[] = {b->d_cams, b->d_views, b->d_work, b->splats, b->keys, b->keys_alt, b->vals, b->vals_alt,
                   b->rects, b->rect_sorted, b->hist, b->piece, b->matrix, b->tile_tot, b->entries, b->ebase,
                   b->run, b->ranges, b->tstate, b->done, b->order, b->flags, b->image, b->dl, b->coef, b->vgrads,
-                  b->band_open, b->walk, b->heavy, b->split, b->prev_walk};
+                  b->band_open, b->walk, b->heavy, b->split, b->prev_walk, b->live_work, b->live_counts};
   for (void* p : ptrs) dfree(p);
   if (b->pinned) (void)hipHostFree(b->pinned);
   delete b;

@@ -128,6 +128,7 @@ struct RasterArgs {
   const ViewDesc* views;
   const TileWork* work;  // [num_work]
   uint32_t num_work;
+  const uint32_t* work_count;  // optional: the number of items of `work`, on the device (replaces num_work)
   const float* splats;
   const uint32_t* order;
   uint64_t order_capacity;  // slots in order; segments ending beyond it are skipped
@@ -186,6 +187,22 @@ struct SplitArgs {
   uint32_t* heavy_count;
 };
 void launch_split_list(hipStream_t s, const SplitArgs& a);
+// The whole-tile items a batched blend launch needs, in tile order: tiles with
+// entries in any of ranges[0 .. num_passes), not flagged done (optional), not
+// split into half-tile items (optional).  A stable compaction of `tiles` into
+// out[0 .. *out_count); counts: [ceil(total_tiles / 256)] scratch.
+struct LiveArgs {
+  const TileWork* tiles;  // [total_tiles]
+  uint32_t total_tiles;
+  const uint32_t* ranges[kMaxPasses];
+  int num_passes;
+  const uint8_t* done;   // optional
+  const uint8_t* split;  // optional
+  uint32_t* counts;
+  TileWork* out;
+  uint32_t* out_count;
+};
+void launch_live_list(hipStream_t s, const LiveArgs& a);
 void launch_forward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid);
 void launch_backward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid);
 // Resident blocks per CU of the persistent forward and backward kernels.
