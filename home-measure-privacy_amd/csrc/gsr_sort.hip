@@ -516,7 +516,8 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t x) {
 }
 
 constexpr uint32_t kEmitQueue = 128;
-constexpr uint32_t kEmitFixedLds = kEmitQueue * 16 + 64 * 4;  // queue records + owner marks
+constexpr uint32_t kOpenWords = 8;  // open-column mask of a band: up to 256 tiles per row
+constexpr uint32_t kEmitFixedLds = kEmitQueue * 16 + 64 * 4 + kOpenWords * 4;  // queue, owner marks, open columns
 // Workgroup L runs on die L mod 8 (eight dies, each with its own L2).  A tile's
 // slots from neighbouring chunks are neighbours in the order, so each die takes
 // kEmitDieRun consecutive chunks of a band: their short runs then meet in one L2
@@ -553,6 +554,14 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
   uint32_t* owner_mark = reinterpret_cast<uint32_t*>(lds_quads + kEmitQueue);  // [64]
   uint64_t* mask = reinterpret_cast<uint64_t*>(owner_mark + 64);             // [ntile]
   uint32_t* cursor = reinterpret_cast<uint32_t*>(mask + ntile);              // [ntile]
+  // Later depth slices: columns of the band with an open (unsaturated) tile in
+  // some row.  Each rectangle is clipped to the open columns before it is
+  // queued, so the pairs of saturated tiles (most of a later slice's pairs on
+  // the bench scene) are not ranked just to be dropped at the store.
+  uint32_t* open_cols = cursor + ntile;  // [kOpenWords]
+  const bool clip = done != nullptr && tx <= 32 * kOpenWords;
+  if (clip && lane < kOpenWords) open_cols[lane] = 0;
+  __syncthreads();
   const uint32_t T = tx * vd.tiles_y;
   const uint32_t* base = matrix + size_t(chunks) * vd.tile_base + size_t(c) * T + size_t(bd.r0) * tx;
   const uint8_t* dn = done ? done + vd.tile_base + size_t(bd.r0) * tx : nullptr;
@@ -572,9 +581,14 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
       cursor[k] = dk[u] ? 0xFFFFFFFFu : bk[u];
       mask[k] = 0;
       open |= dk[u] == 0;
+      if (clip && dk[u] == 0) {
+        const uint32_t col = k % tx;
+        atomicOr(&open_cols[col >> 5], 1u << (col & 31u));
+      }
     }
   }
   if (__ballot(open) == 0) return;  // every tile of the band is saturated
+  __syncthreads();
   const size_t seg = size_t(v) * n;
   const uint32_t begin = pbeg + c * chunk, end = min(pend, begin + chunk);
   const uint64_t ebase = ebase_v[v];
@@ -677,14 +691,26 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
         cgi[k] = cgi[k + 1];
       }
       const uint32_t y0 = max(r.y0, bd.r0), y1 = min(r.y1, bd.r1 - 1);
-      const bool hit = r.x0 <= r.x1 && y0 <= y1;
+      uint32_t x0 = r.x0, x1 = r.x1;
+      bool hit = x0 <= x1 && y0 <= y1;
+      if (clip && hit && (x1 >> 5) <= (x0 >> 5) + 1) {  // wider rectangles keep their columns
+        const uint32_t w0 = x0 >> 5, b0 = w0 << 5;
+        uint64_t m = open_cols[w0];
+        if ((x1 >> 5) > w0) m |= uint64_t(open_cols[w0 + 1]) << 32;
+        m &= (~0ull << (x0 - b0)) & (~0ull >> (63u - (x1 - b0)));
+        hit = m != 0;
+        if (hit) {
+          x0 = b0 + uint32_t(__builtin_ctzll(m));
+          x1 = b0 + uint32_t(63 - __builtin_clzll(m));
+        }
+      }
       const uint64_t hits = __ballot(hit);
       if (hits == 0) continue;
       if (hit) {
         const uint32_t rank =
             __builtin_amdgcn_mbcnt_hi(uint32_t(hits >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(hits), 0u));
-        const uint32_t w = r.x1 - r.x0 + 1;
-        queue[queued + rank] = make_uint4(__umul24(y0 - bd.r0, tx) + r.x0, w, __umul24(w, y1 - y0 + 1), gi);
+        const uint32_t w = x1 - x0 + 1;
+        queue[queued + rank] = make_uint4(__umul24(y0 - bd.r0, tx) + x0, w, __umul24(w, y1 - y0 + 1), gi);
       }
       queued += uint32_t(__popcll(hits));
       if (queued >= 64) flush(64);
