@@ -517,6 +517,8 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t x) {
 
 constexpr uint32_t kEmitQueue = 128;
 constexpr uint32_t kOpenWords = 8;  // open-column mask of a band: up to 256 tiles per row
+constexpr uint32_t kLaterBandScale = 4;  // emit band of later depth slices, in first-slice bands
+constexpr uint32_t kMaxEmitBand = 4096;  // tiles; 12 B of LDS each (gsr_batch_set_emit_band's limit)
 constexpr uint32_t kEmitFixedLds = kEmitQueue * 16 + 64 * 4 + kOpenWords * 4;  // queue, owner marks, open columns
 // Workgroup L runs on die L mod 8 (eight dies, each with its own L2).  A tile's
 // slots from neighbouring chunks are neighbours in the order, so each die takes
@@ -817,18 +819,23 @@ void launch_bin_emit(hipStream_t s, const ViewDesc* views, const ViewDesc* host_
                      uint32_t* order, uint64_t order_capacity, uint32_t* overflow) {
   const uint32_t chunks = slice_chunks(pass, chunk);
   if (num_views == 0 || chunks == 0) return;
-  const BandGeometry g = band_geometry(host_views, num_views, bs.emit_band);
+  // Later slices bin only unsaturated tiles (a third of them on the 32-view
+  // bench scene): their bands are four times taller, so each chunk's
+  // rectangles are scanned by a quarter as many blocks for about the same
+  // pairs (emit 1.90 -> 1.53 ms there; twice and eight times were slower).
+  const uint32_t band = pass.done ? std::min(bs.emit_band * kLaterBandScale, kMaxEmitBand) : bs.emit_band;
+  const BandGeometry g = band_geometry(host_views, num_views, band);
   dim3 tgrid((g.max_tiles + 255) / 256, num_views);
   hipLaunchKernelGGL(bin_colscan_kernel, tgrid, dim3(256), 0, s, views, chunks, pass.ranges, bs.matrix);
   const dim3 grid(chunks, g.max_emit_bands, num_views);
   if (bs.packed)
     hipLaunchKernelGGL(bin_emit_kernel<true>, grid, dim3(64), g.emit_lds, s, views, n, chunk, pass.pbeg, pass.pend,
                        chunks, vals, bs.rect_sorted, pass.done, bs.matrix, pass.ebase, order, order_capacity,
-                       overflow, bs.emit_band);
+                       overflow, band);
   else
     hipLaunchKernelGGL(bin_emit_kernel<false>, grid, dim3(64), g.emit_lds, s, views, n, chunk, pass.pbeg, pass.pend,
                        chunks, vals, bs.rect_sorted, pass.done, bs.matrix, pass.ebase, order, order_capacity,
-                       overflow, bs.emit_band);
+                       overflow, band);
 }
 
 }  // namespace gsr
