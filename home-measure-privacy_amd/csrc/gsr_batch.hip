@@ -271,6 +271,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   a.grads = b->vgrads;
   a.grads_view_stride = uint64_t(b->n) * kRecord;
   a.grads_record = kRecord;
+  a.moments = 1;  // the shape launch below maps them to gradients
   a.tile_batch = b->tile_batch;
   a.tstate = b->num_passes > 1 ? b->tstate : nullptr;
   a.done = b->num_passes > 1 ? b->done : nullptr;
@@ -404,7 +405,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
   launch_backward(s, a, 1, b->grid_bwd);
   mark();
   launch_shape(s, b->d_cams, b->V, gs, sh, b->deg, b->n, b->vgrads, uint64_t(b->n) * kRecord, shape, sh_grad, grads,
-               1, prof && prof->last ? prof->touched : nullptr, kRecord, b->valid_mask, stats, b->flags);
+               1, prof && prof->last ? prof->touched : nullptr, kRecord, b->valid_mask, stats, b->flags, 1);
   mark();
 }
 

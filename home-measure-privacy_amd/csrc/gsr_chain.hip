@@ -14,6 +14,7 @@ namespace gsr {
 
 namespace {
 
+template <bool kMoments>
 __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict__ cams, uint32_t num_views,
                                                     const float* __restrict__ gs, const float* __restrict__ sh,
                                                     int deg, uint32_t n, float* __restrict__ grads,
@@ -58,14 +59,29 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
   float gmean[3] = {0.f, 0.f, 0.f};
   float gcol[3] = {0.f, 0.f, 0.f};
   float gop = 0.0f;
+  // Moment records: d opLogit = (1 - op) m3, op the sigmoid of gpu.js:85 as
+  // the projection computes it.
+  const float op_scale = kMoments ? 1.0f - 1.0f / (1.0f + expf(-pb.w)) : 1.0f;
   // Most (Gaussian, view) records are zero, so the walk over views is bound by
   // load latency: the records of kAhead views are loaded before any is used.
   constexpr uint32_t kAhead = 4;
-  auto view_step = [&](uint32_t v, const float (&gr)[9]) __attribute__((always_inline)) {
+  // STATS (gpu.js:491-497) of one view: |(du, dv)| where the Gaussian projects
+  // validly (valid_mask), in view order.
+  auto add_stats = [&](bool valid, float gu, float gv) __attribute__((always_inline)) {
+    if (!valid) return;
+    st_norm += sqrtf(gu * gu + gv * gv);
+    st_count += 1.0f;
+  };
+  // A zero (du, dv): the norm adds +0, which leaves the sum as it is.
+  auto add_zero_stats = [&](bool valid) __attribute__((always_inline)) { st_count += valid ? 1.0f : 0.0f; };
+  auto view_step = [&](uint32_t v, const float (&gr)[9], bool valid) __attribute__((always_inline)) {
     float4* r = reinterpret_cast<float4*>(grads + size_t(v) * stride + size_t(i) * record);
     gcol[0] += gr[0]; gcol[1] += gr[1]; gcol[2] += gr[2];
-    gop += gr[3];
+    if (kMoments) gop = fmaf(op_scale, gr[3], gop);
+    else gop += gr[3];
+    if (!kMoments) add_stats(valid, gr[4], gr[5]);
     bool screen = gr[4] != 0.0f || gr[5] != 0.0f || gr[6] != 0.0f || gr[7] != 0.0f || gr[8] != 0.0f;
+    if (kMoments && !screen) add_zero_stats(valid);
     bool colour = deg > 0 && (gr[0] != 0.0f || gr[1] != 0.0f || gr[2] != 0.0f);
     const bool any = screen || gr[0] != 0.0f || gr[1] != 0.0f || gr[2] != 0.0f || gr[3] != 0.0f;
     if (consume && any) {
@@ -78,12 +94,18 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
         atomicAdd(touched + v, uint32_t(__popcll(m)));
     }
     if (!screen && !colour) return;
+    // A screen record with no valid projection here (it cannot have one: the
+    // blends only see splats that projected) adds a zero norm.
+    const bool stats_open = kMoments && screen;
     const CamF cam = load_cam(cams[v]);
     float p[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
       p[k] = cam.V[k][0] * mean[0] + cam.V[k][1] * mean[1] + cam.V[k][2] * mean[2] + cam.t[k];
-    if (p[2] < 0.2f) return;
+    if (p[2] < 0.2f) {
+      if (stats_open) add_zero_stats(valid);
+      return;
+    }
     // Camera-frame covariance, as project_cov.
     float T[3][3], C[3][3];
 #pragma unroll
@@ -108,11 +130,23 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
     const float cb = j0[0] * Cj1[0] + j0[1] * Cj1[1] + j0[2] * Cj1[2];
     const float cc = j1[0] * Cj1[0] + j1[1] * Cj1[1] + j1[2] * Cj1[2] + 0.3f;
     const float D = ca * cc - cb * cb;
-    if (!(D > 0.0f)) return;
+    if (!(D > 0.0f)) {
+      if (stats_open) add_zero_stats(valid);
+      return;
+    }
     if (screen) {
-      const float gu = gr[4], gv = gr[5], gA = gr[6], gB = gr[7], gC = gr[8];
       // Conic (A, B, C) = (c, -b, a) / D.
       const float iD = 1.0f / D, iD2 = iD * iD;
+      float gu = gr[4], gv = gr[5], gA = gr[6], gB = gr[7], gC = gr[8];
+      if (kMoments) {
+        const float A = cc * iD, B = -cb * iD, C = ca * iD;
+        gu = fmaf(A, gr[4], B * gr[5]);
+        gv = fmaf(B, gr[4], C * gr[5]);
+        gA = -0.5f * gr[6];
+        gB = -gr[7];
+        gC = -0.5f * gr[8];
+        add_stats(valid, gu, gv);
+      }
       const float g_a = (-gA * cc * cc + gB * cb * cc - gC * cb * cb) * iD2;
       const float g_b = (2.0f * gA * cb * cc + 2.0f * gC * ca * cb - 2.0f * gB * cb * cb) * iD2 - gB * iD;
       const float g_c = (-gA * cb * cb + gB * ca * cb - gC * ca * ca) * iD2;
@@ -195,11 +229,8 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
 #pragma unroll
     for (uint32_t k = 0; k < kAhead; ++k) {
       if (v0 + k >= num_views) continue;
-      if (stats && ((valid_mask[size_t(v0 + k) * mask_words + i / 64] >> (i % 64)) & 1u)) {
-        st_norm += sqrtf(rec[k][4] * rec[k][4] + rec[k][5] * rec[k][5]);
-        st_count += 1.0f;
-      }
-      view_step(v0 + k, rec[k]);
+      const bool valid = stats && ((valid_mask[size_t(v0 + k) * mask_words + i / 64] >> (i % 64)) & 1u);
+      view_step(v0 + k, rec[k], valid);
     }
   }
   if (stats) {
@@ -257,12 +288,17 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
 void launch_shape(hipStream_t s, const gsr_camera* cams, uint32_t num_views, const float* gs, const float* sh,
                   int sh_degree, uint32_t n, float* grads, uint64_t grads_view_stride, float* shape,
                   float* sh_grad, float* grads_out, int consume, uint32_t* touched, uint32_t record,
-                  const uint64_t* valid_mask, float* stats, const uint32_t* stats_guard) {
+                  const uint64_t* valid_mask, float* stats, const uint32_t* stats_guard, int moments) {
   if (n == 0) return;
   if (!valid_mask) stats = nullptr;
-  hipLaunchKernelGGL(shape_kernel, dim3((n + 255) / 256), dim3(256), 0, s, cams, num_views, gs, sh, sh_degree, n,
-                     grads, grads_view_stride, shape, sh_grad, grads_out, consume, touched, record, valid_mask,
-                     stats, stats_guard);
+  if (moments)
+    hipLaunchKernelGGL(shape_kernel<true>, dim3((n + 255) / 256), dim3(256), 0, s, cams, num_views, gs, sh,
+                       sh_degree, n, grads, grads_view_stride, shape, sh_grad, grads_out, consume, touched, record,
+                       valid_mask, stats, stats_guard);
+  else
+    hipLaunchKernelGGL(shape_kernel<false>, dim3((n + 255) / 256), dim3(256), 0, s, cams, num_views, gs, sh,
+                       sh_degree, n, grads, grads_view_stride, shape, sh_grad, grads_out, consume, touched, record,
+                       valid_mask, stats, stats_guard);
 }
 
 }  // namespace gsr

@@ -167,7 +167,19 @@ struct RasterArgs {
   // Nonzero: no hit-box culling.  Every entry is evaluated at every pixel of
   // its tile, as alphaAt does; results must not change (verification knob).
   int no_cull;
+  // Backward, nonzero: the records receive the entry's pixel moments instead of
+  // its gradient terms (record layout kMomentSlots); launch_shape with
+  // moments != 0 turns them into gradients.
+  int moments;
 };
+// Moment record of one (Gaussian, view): with dP = dLoss/dpower and (dx, dy) the
+// pixel centre less the splat centre (u, v), summed over the pixels it blends:
+//   0-2  dLoss/dcolour (r, g, b)      3  sum dP
+//   4    sum dP dx                    5  sum dP dy
+//   6    sum dP dx^2                  7  sum dP dx dy      8  sum dP dy^2
+// The screen gradients are linear in them (gpu.js:266-281): d opLogit = (1 -
+// op) m3, du = A m4 + B m5, dv = B m4 + C m5, dA = -m6 / 2, dB = -m7, dC = -m8 / 2.
+constexpr int kMomentSlots = 9;
 // done[g] = (no band of tile g open) with assign != 0 (after the first pass, which
 // blends every tile), else done[g] |= (no band open).
 void launch_done_update(hipStream_t s, uint32_t total_tiles, const uint8_t* band_open, uint8_t* done, int assign);
@@ -223,7 +235,9 @@ void launch_zero(hipStream_t s, void* p, size_t bytes);
 // which leaves the per-view accumulators clear for the next backward.
 // touched (optional, [num_views]) counts the Gaussians with any gradient per view.
 // record: floats per per-view gradient record (16, or 12 for the batch's packed
-// records; the nine terms sit at 0-8 either way).
+// records; the nine terms sit at 0-8 either way).  moments: the records hold
+// pixel moments (RasterArgs::moments), mapped to gradients with the view's conic
+// and the Gaussian's opacity.
 // stats (optional, [n][2], with valid_mask from launch_project): STATS
 // (gpu.js:483-497) for every view in turn, stats[i] += (|(du, dv)|, 1) where
 // Gaussian i projects validly in the view; left unchanged when the device word
@@ -232,7 +246,7 @@ void launch_shape(hipStream_t s, const gsr_camera* cams, uint32_t num_views, con
                   int sh_degree, uint32_t n, float* grads, uint64_t grads_view_stride, float* shape,
                   float* sh_grad, float* grads_out, int consume, uint32_t* touched = nullptr, uint32_t record = 16,
                   const uint64_t* valid_mask = nullptr, float* stats = nullptr,
-                  const uint32_t* stats_guard = nullptr);
+                  const uint32_t* stats_guard = nullptr, int moments = 0);
 void launch_adam_sh(hipStream_t s, const gsr_step& step, float lr_sh_rest, float* gs, float* sh, const float* grads,
                     const float* shape, const float* sh_grad, float* moments);
 void launch_adam(hipStream_t s, const gsr_step& step, float* gs, const float* grads, const float* shape,

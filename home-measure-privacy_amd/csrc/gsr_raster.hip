@@ -523,16 +523,47 @@ __device__ inline int reduce9_slot(uint32_t lane) {
   return lane == 2 ? 8 : -1;
 }
 
+// Wave sums of the nine pixel moments (kMomentSlots) from each lane's six sums
+// over its pixels (colour terms, S0 = sum dP, S1 = sum dP dy, S2 = sum dP dy^2).
+// The lanes of a column share dx, so the x-weighted moments are formed once the
+// first fold has paired rows r and r + 2: one fold32 per two raw sums, two
+// multiplies for dx S0, dx S1 and dx^2 S0, then the fold16s and row sums as in
+// reduce9.  dx^2 S0 skips the fold16 (it has no partner): its rows 0 and 1 are
+// summed, and row 1 adds row 0's total by row_bcast:15, a DPP move at half the
+// cost of a swap.  Lane (16 row + col) ends with moment moment_slot(lane).
+__device__ inline float reduce_moments(float gc0, float gc1, float gc2, float s0, float s1, float s2, float dx,
+                                       uint32_t lane) {
+  const float h0 = fold32(gc0, gc1), h1 = fold32(gc2, s2), h2 = fold32(s0, s1);
+  const float h3 = dx * h2;  // lanes 0-31: dx S0, lanes 32-63: dx S1
+  const float h4 = dx * h3;  // lanes 0-31: dx^2 S0 (lanes 32-63 unused)
+  const float k0 = row_sum(fold16(h0, h1));  // rows: colour 0, colour 2, colour 1, S2
+  const float k1 = row_sum(fold16(h2, h3));  // rows: S0, dx S0, S1, dx S1
+  const float r2 = row_sum(h4);
+  // row_bcast:15 (0x142) into row 1 only (row mask 0x2): lane 15's value.
+  const float k2 = r2 + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(r2), 0x142, 0x2, 0xF, false));
+  const uint32_t col = lane & 15;
+  return col == 0 ? k0 : (col == 1 ? k1 : k2);  // k2: dx^2 S0 in row 1
+}
+
+// Moment slot summed into lane `lane` by reduce_moments, or -1 for none.
+__device__ inline int moment_slot(uint32_t lane) {
+  const uint32_t row = lane >> 4, col = lane & 15;
+  constexpr int kCol0[4] = {0, 2, 1, 8}, kCol1[4] = {3, 4, 5, 7};
+  if (col == 0) return kCol0[row];
+  if (col == 1) return kCol1[row];
+  return lane == 18 ? 6 : -1;
+}
+
 __device__ inline float sign_f(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
 
-template <bool kNoCull>
-__device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[4], uint32_t* gi_lds) {
+template <bool kNoCull, bool kMoments>
+__device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, float4 (*lds)[4], uint64_t* rec_lds) {
   const uint32_t lane = __lane_id();
   Tile t;
   if (!load_tile(a, w, t)) return;
   const uint32_t x = t.x0 + (lane & 15);
   const float px = float(x) + 0.5f;
-  const int slot = reduce9_slot(lane);
+  const int slot = kMoments ? moment_slot(lane) : reduce9_slot(lane);
   // Per pixel: dL/dC (d) and bd = d . (the colour behind the entries walked so
   // far): d . image less d . c times each walked entry's weight.  The behind
   // term of gpu.js:266-268 needs the behind colour only dotted with d, and d . c
@@ -585,7 +616,7 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
     const Box sbox = active_box(t.x0, t.y0, T);
     __syncthreads();
     const uint32_t sb = stage<kNoCull>(lds, lane, nb, band0, sbox) & items;
-    gi_lds[lane] = nb.gi;
+    rec_lds[lane] = uint64_t(nb.gi) * record;  // the record's offset, multiplied once per batch
     __syncthreads();
     // Unconditional: past the end the loads read the list's last entry and the
     // batch is flagged out (and never staged).
@@ -639,31 +670,37 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
         s2 = fmaf(dPdy, dy, s2);
       }
       if (!hit) continue;  // wave-uniform: set only for bands with a hit
-      // gpu.js:270-281 summed over the lane's pixels: with S0 = sum dP, S1 = sum
-      // dP dy and S2 = sum dP dy^2 (dP = dLoss/dpower), d opLogit = (1 - op) S0,
-      // du = A dx S0 + B S1, dv = B dx S0 + C S1, dA = -dx^2/2 S0, dB = -dx S1,
-      // dC = -S2/2 (dx is the lane's, shared by its pixels).
-      const float g[9] = {gc0, gc1, gc2, s0 * (1.0f - op), fmaf(s1, B, s0 * (A * dx)), fmaf(s1, C, s0 * (B * dx)),
-                          s0 * (-0.5f * dx * dx), -(s1 * dx), -0.5f * s2};
-      const float val = reduce9(g, lane);
-      if (slot >= 0 && val != 0.0f) unsafeAtomicAdd(grads + size_t(gi_lds[j]) * record + slot, val);
+      float val;
+      if (kMoments) {
+        val = reduce_moments(gc0, gc1, gc2, s0, s1, s2, dx, lane);
+      } else {
+        // gpu.js:270-281 summed over the lane's pixels: with S0 = sum dP, S1 =
+        // sum dP dy and S2 = sum dP dy^2 (dP = dLoss/dpower), d opLogit = (1 -
+        // op) S0, du = A dx S0 + B S1, dv = B dx S0 + C S1, dA = -dx^2/2 S0, dB =
+        // -dx S1, dC = -S2/2 (dx is the lane's, shared by its pixels).
+        const float g[9] = {gc0, gc1, gc2, s0 * (1.0f - op), fmaf(s1, B, s0 * (A * dx)), fmaf(s1, C, s0 * (B * dx)),
+                            s0 * (-0.5f * dx * dx), -(s1 * dx), -0.5f * s2};
+        val = reduce9(g, lane);
+      }
+      if (slot >= 0 && val != 0.0f) unsafeAtomicAdd(grads + rec_lds[j] + slot, val);
     }
   }
   }
 }
 
-template <bool kNoCull>
+template <bool kNoCull, bool kMoments>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void backward_kernel(RasterArgs a) {
   __shared__ float4 lds[64][4];
-  __shared__ uint32_t gi_lds[64];
-  backward_tile<kNoCull>(a, blockIdx.x, lds, gi_lds);
+  __shared__ uint64_t rec_lds[64];
+  backward_tile<kNoCull, kMoments>(a, blockIdx.x, lds, rec_lds);
 }
 
-template <bool kNoCull>
+template <bool kNoCull, bool kMoments>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void backward_persistent_kernel(RasterArgs a) {
   __shared__ float4 lds[64][4];
-  __shared__ uint32_t gi_lds[64];
-  for_each_item(a, [&](uint32_t w) __attribute__((always_inline)) { backward_tile<kNoCull>(a, w, lds, gi_lds); });
+  __shared__ uint64_t rec_lds[64];
+  for_each_item(a, [&](uint32_t w)
+                    __attribute__((always_inline)) { backward_tile<kNoCull, kMoments>(a, w, lds, rec_lds); });
 }
 
 // ---- loss gradient (gpu.js:336-409) ---------------------------------------------
@@ -1028,25 +1065,30 @@ void launch_forward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t
   }
 }
 
-template <bool kNoCull>
+template <bool kNoCull, bool kMoments>
 void launch_backward_variant(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid) {
   if (persistent)
-    hipLaunchKernelGGL(backward_persistent_kernel<kNoCull>, dim3(grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((backward_persistent_kernel<kNoCull, kMoments>), dim3(grid), dim3(64), 0, s, a);
   else
-    hipLaunchKernelGGL(backward_kernel<kNoCull>, dim3(a.num_work), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((backward_kernel<kNoCull, kMoments>), dim3(a.num_work), dim3(64), 0, s, a);
 }
 
 void launch_backward(hipStream_t s, const RasterArgs& a, int persistent, uint32_t grid) {
   if (a.num_work == 0) return;
-  if (a.no_cull) launch_backward_variant<true>(s, a, persistent, grid);
-  else launch_backward_variant<false>(s, a, persistent, grid);
+  if (a.moments) {
+    if (a.no_cull) launch_backward_variant<true, true>(s, a, persistent, grid);
+    else launch_backward_variant<false, true>(s, a, persistent, grid);
+  } else {
+    if (a.no_cull) launch_backward_variant<true, false>(s, a, persistent, grid);
+    else launch_backward_variant<false, false>(s, a, persistent, grid);
+  }
 }
 
 void raster_occupancy(int* forward_per_cu, int* backward_per_cu) {
   int f = 0, b = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&f, forward_persistent_kernel<false, false>, 64, 0) != hipSuccess)
     f = 8;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, backward_persistent_kernel<false>, 64, 0) != hipSuccess) b = 8;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, backward_persistent_kernel<false, true>, 64, 0) != hipSuccess) b = 8;
   *forward_per_cu = f;
   *backward_per_cu = b;
 }
