@@ -17,9 +17,14 @@
 // pixel.  The colour behind an entry enters only as d . behind (d = dL/dC): it
 // starts as d . image and loses d . c times each entry's weight as the walk
 // passes it.  Per entry, the four pixels of a lane are summed in
-// registers, the nine terms are reduced over the wave together (cross-half lane
+// registers, nine values are reduced over the wave together (cross-half lane
 // swaps, then DPP within rows), and nine lanes issue one float atomic each into
-// the 16-float gradient record.
+// the entry's record.  The values are either the reference's nine gradient terms
+// (the 16-float record of the standalone backward) or, in the batched step, nine
+// pixel moments the terms are linear in, so the entry's conic and opacity are
+// applied once per (Gaussian, view) by the shape kernel instead of in every
+// lane: 5 lane swaps per entry instead of 8 (backward 4.44 -> 4.13 ms on the
+// 32-view 1M-Gaussian step).
 #include "gsr_internal.h"
 #include "gsr_math.h"
 
