@@ -31,11 +31,11 @@ hipError_t ctx_malloc(gsr_ctx* ctx, void** p, size_t bytes);
 }  // namespace gsr
 
 constexpr uint32_t kLiveBase = 32;
-constexpr uint32_t kQueueBase = 48;
+constexpr uint32_t kQueueBase = 64;
 // Per-view gradient records hold the nine screen-space terms at 0-8, packed 12
 // floats apart (48 bytes: wave reads are contiguous, no padding fetched).
 constexpr uint32_t kRecord = 12;
-constexpr uint32_t kFlagWords = kQueueBase + 9 * (gsr::kMaxPasses + 1);
+constexpr uint32_t kFlagWords = kQueueBase + 9 * (gsr::kMaxPasses + 1) * gsr::kQueueStride;
 
 struct gsr_batch {
   gsr_ctx* ctx = nullptr;
@@ -89,7 +89,8 @@ struct gsr_batch {
   // [0] overflow, [16 + p] split-item count of forward pass p, [16 + kMaxPasses]
   // the backward's; [kLiveBase + p] live-list length of forward pass p,
   // [kLiveBase + kMaxPasses] the backward's; from kQueueBase, 9 work counters
-  // per forward pass, then the backward's (a shared one and one per die part).
+  // per forward pass, then the backward's (a shared one and one per die part),
+  // kQueueStride words apart.
   uint32_t* flags = nullptr;
   float* image = nullptr;
   float* dl = nullptr;
@@ -301,7 +302,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
                     b->flags);
     if (prof) (void)hipEventRecord(e2, s);
     a.pass = int(p);
-    a.queue = b->flags + kQueueBase + 9 * p;
+    a.queue = b->flags + kQueueBase + 9 * p * kQueueStride;
     const uint32_t split_cost = p ? b->split_later : b->split_first;
     a.split = split_cost ? b->split : nullptr;
     a.heavy = b->heavy;
@@ -401,7 +402,7 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
     a.work = b->live_work;
     a.work_count = la.out_count;
   }
-  a.queue = b->flags + kQueueBase + 9 * kMaxPasses;
+  a.queue = b->flags + kQueueBase + 9 * kMaxPasses * kQueueStride;
   launch_backward(s, a, 1, b->grid_bwd);
   mark();
   launch_shape(s, b->d_cams, b->V, gs, sh, b->deg, b->n, b->vgrads, uint64_t(b->n) * kRecord, shape, sh_grad, grads,

@@ -443,7 +443,7 @@ __device__ inline uint32_t pop_work(uint32_t* queue, uint32_t n) {
 // So the whole tiles are cut into kXcds contiguous parts, part d served first to
 // the workgroups of die d from its own counter; a die that runs out helps the
 // others in turn.  The split tiles' items come first, from a shared counter.
-// queue: [0] the split items' counter, [1 + d] part d's.
+// queue: [0] the split items' counter, [kQueueStride (1 + d)] part d's.
 constexpr uint32_t kXcds = 8;
 
 template <class Blend>
@@ -462,7 +462,7 @@ __device__ __forceinline__ void for_each_item(const RasterArgs& a, Blend&& blend
     const uint32_t lo = nh + uint32_t(uint64_t(num_work) * part / kXcds);
     const uint32_t hi = nh + uint32_t(uint64_t(num_work) * (part + 1) / kXcds);
     for (;;) {
-      const uint32_t w0 = lo + pop_work(a.queue + 1 + part, a.tile_batch);
+      const uint32_t w0 = lo + pop_work(a.queue + kQueueStride * (1 + part), a.tile_batch);
       if (w0 >= hi) break;
       const uint32_t w1 = min(hi, w0 + a.tile_batch);
       for (uint32_t w = w0; w < w1; ++w) blend(w);
@@ -666,8 +666,10 @@ __device__ __forceinline__ void backward_tile(const RasterArgs& a, uint32_t w, f
         gc2 = fmaf(db[i], wgt, gc2);
         float dA = fmaf(Tk, dc, -(bd[i] * __builtin_amdgcn_rcpf(oma)));
         // gpu.js:270-281: opacity logit and screen (u, v, conic) gradients, only
-        // while the 0.99 clamp is not active.
-        dA = (h && alpha < kAlphaMax) ? dA : 0.0f;
+        // while the 0.99 clamp is not active.  A missed pixel of the band has
+        // alpha 0 and a finite dA (1 - alpha = 1), so its dP is zero without a
+        // test of its own.
+        dA = alpha < kAlphaMax ? dA : 0.0f;
         const float dP = dA * alpha;
         const float dPdy = dP * dy;
         s0 += dP;
