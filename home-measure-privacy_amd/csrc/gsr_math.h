@@ -197,6 +197,8 @@ __device__ inline void sh_basis_grad(int deg, const float d[3], float G[16][3]) 
 
 // colour_c = sh[c] + sum_{k>=1} Y_k(dir) sh[3k + c], dir = normalise(mean - centre).
 __device__ inline void sh_colour(int deg, const float sh[48], const float mean[3], const CamF& cam, float rgb[3]) {
+  // The colour feeds no discrete rule of the blend, so its sum may fuse.
+#pragma clang fp contract(fast)
   float dir[3] = {mean[0] - cam.centre[0], mean[1] - cam.centre[1], mean[2] - cam.centre[2]};
   float len = sqrtf(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
   float inv = len > 0.0f ? 1.0f / len : 0.0f;

@@ -363,6 +363,36 @@ def test_gradients_end_to_end_crop(ctx):
     r.close()
 
 
+def test_gradients_end_to_end_10k_256px_follow_reference_order(ctx):
+    """gradients() (gpu.js:826-839) on the 10k-Gaussian SH0 256x256 frame from the
+    packed parameters, against the float32 restatement of the reference run from
+    the same parameters (its own projection, tileLists and blend) and the FP64
+    oracle.  The projection follows the reference's operation order, so every
+    discrete rule of the blend (1/255 skip, 0.99 clamp, T stop, square cull)
+    decides as in the restatement: a centre one ulp off flips the 1/255 skip at a
+    pixel of this frame and moves the shape gradients by 1.5e-4 against FP64."""
+    sc = scenes.config(0)
+    cam = sc.cams[0]
+    rng = np.random.default_rng(0)
+    dl = (rng.standard_normal((cam.height, cam.width, 3)) / (cam.pixels * 3)).astype(np.float32)
+    r = Renderer()
+    out = r.gradients(sc.gs, cam, None, dl=dl)
+    r.close()
+    sp32, lst32, _, image32, state32 = O.render(cam, sc.gs, None, 0, np.float32)
+    assert out["entries"] == lst32.size
+    d = np.abs(out["image"].astype(np.float64) - image32)
+    log(f"  image vs float32 restatement: rel {rel_l2(out['image'], image32):.2e} max {d.max():.2e}")
+    # No rule flips: every pixel within a few ulp of the restatement's.
+    assert d.max() < 1e-5
+    image, grads, shape, _ = O.gradients(cam, sc.gs.astype(np.float64), dl.astype(np.float64), None, 0, np.float64)
+    res = {"image": rel_l2(out["image"], image), "grads": rel_l2(out["grads"][:, :9], grads[:, :9]),
+           "means": rel_l2(out["shape"][:, 0:3], shape[:, 0:3]), "log_scales": rel_l2(out["shape"][:, 3:6], shape[:, 3:6]),
+           "quaternions": rel_l2(out["shape"][:, 6:10], shape[:, 6:10])}
+    log(f"  end-to-end rel L2 vs FP64: {res}")
+    for k, v in res.items():
+        assert v < 1e-4, k
+
+
 def test_gradients_match_central_differences_64_params(ctx):
     """Differentiation check on the 1M-Gaussian SH3 scene: GPU gradients (4k Gaussians, 128x128 crop)
     against FP64 central differences of the loss on 64 random parameters (means,
