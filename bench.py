@@ -14,6 +14,7 @@ Prints one JSON line on stdout; progress goes to stderr.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -37,13 +38,34 @@ STAGE_KNOBS = {"forward": (0, 2, 3), "backward": (0, 4), "bin_sort": (1, 5), "pr
 PROFILE = os.path.join(ROOT, "profiles", "traffic_batched_1M_32views.json")
 
 
+def kernel_digest():
+    """Digest of the kernel sources and their build flags (csrc/ and gsr/build.py):
+    a committed counter profile describes the build whose digest it records."""
+    pkg = os.path.join(ROOT, "home-measure-privacy_amd")
+    files = [os.path.join(d, f) for d, _, fs in os.walk(os.path.join(pkg, "csrc")) for f in fs]
+    files.append(os.path.join(pkg, "gsr", "build.py"))
+    h = hashlib.sha256()
+    for f in sorted(files):
+        h.update(os.path.relpath(f, ROOT).encode())
+        h.update(open(f, "rb").read())
+    return h.hexdigest()[:16]
+
+
 def stage_profile(stage, tuning):
     """The committed counter profile (HBM traffic, VALU instructions per step,
-    dual-issued fraction) of `stage` at the knobs of `tuning` it reads, or None."""
+    dual-issued fraction) of `stage` at the knobs of `tuning` it reads, and a note
+    (None, or why there is no profile: none at these knobs, or one of another
+    kernel build)."""
     if not os.path.exists(PROFILE):
-        return None
+        return None, "no committed counter profile"
+    prof = json.load(open(PROFILE))
+    if prof.get("kernels") != kernel_digest():
+        return None, f"the committed counter profile is of another kernel build ({prof.get('kernels')})"
     key = ",".join(str(tuning[i]) for i in STAGE_KNOBS[stage])
-    return json.load(open(PROFILE)).get("by_knobs", {}).get(stage, {}).get(key)
+    entry = prof.get("by_knobs", {}).get(stage, {}).get(key)
+    return entry, None if entry else f"no counter profile of {stage} at these knobs"
+
+
 SH_BYTES = 192
 
 
@@ -504,10 +526,8 @@ def main():
     traffic, traffic_note = None, None
     full = (n, V) == (1_000_000, 32)
     if full:
-        prof = stage_profile(dominant, tuning)
-        if prof is None:
-            traffic_note = f"no FETCH_SIZE / WRITE_SIZE profile of {dominant} at these knobs"
-        else:
+        prof, traffic_note = stage_profile(dominant, tuning)
+        if prof is not None:
             traffic = round(prof["traffic"])
 
     # Compute roofline of the slower blend kernel: its VALU instructions per step
@@ -516,10 +536,8 @@ def main():
     compute, compute_note = None, None
     blend = max(("forward", "backward"), key=lambda k: timed[k])
     if full:
-        prof = stage_profile(blend, tuning)
-        if prof is None:
-            compute_note = f"no SQ_INSTS_VALU profile of {blend} at these knobs"
-        else:
+        prof, compute_note = stage_profile(blend, tuning)
+        if prof is not None:
             vpeak = measure_valu_peak(ctx)
             rate = prof["valu"] / (timed[blend] * 1e-3)
             compute = {"bound": "valu-issue", "kernel": blend, "unit": "wave64 VALU instructions/s",
