@@ -32,9 +32,10 @@ hipError_t ctx_malloc(gsr_ctx* ctx, void** p, size_t bytes);
 
 constexpr uint32_t kLiveBase = 32;
 constexpr uint32_t kQueueBase = 64;
-// Per-view gradient records hold the nine screen-space terms at 0-8, packed 12
-// floats apart (48 bytes: wave reads are contiguous, no padding fetched).
-constexpr uint32_t kRecord = 12;
+// Per-view gradient records hold the backward's nine pixel moments, packed 9
+// floats (36 bytes) apart: the chain reads every record of every view, whole
+// cache lines at a time, so padding would be fetched with them.
+constexpr uint32_t kRecord = 9;
 constexpr uint32_t kFlagWords = kQueueBase + 9 * (gsr::kMaxPasses + 1) * gsr::kQueueStride;
 
 struct gsr_batch {

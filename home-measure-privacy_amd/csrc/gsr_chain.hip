@@ -14,6 +14,12 @@ namespace gsr {
 
 namespace {
 
+// Three floats at 4-byte alignment (the batch's 9-float moment records lie 36
+// bytes apart): one 12-byte access.
+struct __attribute__((aligned(4))) Float3u {
+  float x, y, z;
+};
+
 template <bool kMoments>
 __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict__ cams, uint32_t num_views,
                                                     const float* __restrict__ gs, const float* __restrict__ sh,
@@ -75,7 +81,7 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
   // A zero (du, dv): the norm adds +0, which leaves the sum as it is.
   auto add_zero_stats = [&](bool valid) __attribute__((always_inline)) { st_count += valid ? 1.0f : 0.0f; };
   auto view_step = [&](uint32_t v, const float (&gr)[9], bool valid) __attribute__((always_inline)) {
-    float4* r = reinterpret_cast<float4*>(grads + size_t(v) * stride + size_t(i) * record);
+    float* rp = grads + size_t(v) * stride + size_t(i) * record;
     gcol[0] += gr[0]; gcol[1] += gr[1]; gcol[2] += gr[2];
     if (kMoments) gop = fmaf(op_scale, gr[3], gop);
     else gop += gr[3];
@@ -85,8 +91,15 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
     bool colour = deg > 0 && (gr[0] != 0.0f || gr[1] != 0.0f || gr[2] != 0.0f);
     const bool any = screen || gr[0] != 0.0f || gr[1] != 0.0f || gr[2] != 0.0f || gr[3] != 0.0f;
     if (consume && any) {
-      const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-      r[0] = z; r[1] = z; r[2] = z;
+      if (kMoments) {
+        Float3u* r = reinterpret_cast<Float3u*>(rp);
+        const Float3u z{0.f, 0.f, 0.f};
+        r[0] = z; r[1] = z; r[2] = z;
+      } else {
+        float4* r = reinterpret_cast<float4*>(rp);
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        r[0] = z; r[1] = z; r[2] = z;
+      }
     }
     if (touched) {  // profiling only: one atomic per wave
       const uint64_t m = __ballot(any);
@@ -220,11 +233,20 @@ __global__ __launch_bounds__(256) void shape_kernel(const gsr_camera* __restrict
     for (uint32_t k = 0; k < kAhead; ++k) {
       // Past the last view the last view's record is loaded again and not used.
       const uint32_t v = min(v0 + k, num_views - 1);
-      const float4* r = reinterpret_cast<const float4*>(grads + size_t(v) * stride + size_t(i) * record);
-      const float4 a = r[0], b = r[1];
-      rec[k][0] = a.x; rec[k][1] = a.y; rec[k][2] = a.z; rec[k][3] = a.w;
-      rec[k][4] = b.x; rec[k][5] = b.y; rec[k][6] = b.z; rec[k][7] = b.w;
-      rec[k][8] = reinterpret_cast<const float*>(r)[8];
+      const float* rp = grads + size_t(v) * stride + size_t(i) * record;
+      if (kMoments) {  // 9-float moment records, 4-byte aligned: three 12-byte loads
+        const Float3u* r = reinterpret_cast<const Float3u*>(rp);
+        const Float3u a = r[0], b = r[1], c = r[2];
+        rec[k][0] = a.x; rec[k][1] = a.y; rec[k][2] = a.z;
+        rec[k][3] = b.x; rec[k][4] = b.y; rec[k][5] = b.z;
+        rec[k][6] = c.x; rec[k][7] = c.y; rec[k][8] = c.z;
+      } else {  // the reference's 16-float records
+        const float4* r = reinterpret_cast<const float4*>(rp);
+        const float4 a = r[0], b = r[1];
+        rec[k][0] = a.x; rec[k][1] = a.y; rec[k][2] = a.z; rec[k][3] = a.w;
+        rec[k][4] = b.x; rec[k][5] = b.y; rec[k][6] = b.z; rec[k][7] = b.w;
+        rec[k][8] = rp[8];
+      }
     }
 #pragma unroll
     for (uint32_t k = 0; k < kAhead; ++k) {
