@@ -30,11 +30,13 @@ from gsr import lib, scenes  # noqa: E402
 METRIC = "views/s fwd+bwd, 1M Gaussians: 32-view batched vs 1-view plain; % HBM peak"
 HBM_PEAK_GBS = 8000.0
 # The tuning knobs (tile_batch, chunk, split_first, split_later, split_backward,
-# emit_band) each stage's kernels read: a stage's committed counter profile holds
-# for every tuning that agrees on these.  The forward splits its heavy tiles by
-# split_first / split_later, the backward by split_backward; binning reads chunk
-# and emit_band; projection and the chain read none.
-STAGE_KNOBS = {"forward": (0, 2, 3), "backward": (0, 4), "bin_sort": (1, 5), "project": (), "chain": ()}
+# emit_band, depth-slice passes, growth) each stage's kernels read: a stage's
+# committed counter profile holds for every tuning that agrees on these.  The
+# forward splits its heavy tiles by split_first / split_later, the backward by
+# split_backward; binning reads chunk and emit_band; the depth slices decide
+# what the binning and both blends see; projection and the chain read none.
+STAGE_KNOBS = {"forward": (0, 2, 3, 6, 7), "backward": (0, 4, 6, 7), "bin_sort": (1, 5, 6, 7), "project": (),
+               "chain": ()}
 PROFILE = os.path.join(ROOT, "profiles", "traffic_batched_1M_32views.json")
 
 
@@ -362,6 +364,8 @@ def tune(args, batch, gsb, shb, tgt, grads, shape, shg):
         batch.set_tuning([int(x) for x in args.tune.split(",")])
     elif not args.no_autotune:
         batch.autotune(gsb, shb, tgt, 0.0, grads, shape, shg)
+        if args.passes:  # explicit depth slices win over the autotuned ones
+            batch.set_passes(args.passes, args.growth)
     tuning = batch.tuning()
     log("tuning: " + ",".join(str(x) for x in tuning))
     return tuning
@@ -454,8 +458,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--tune", default="", help="tile_batch,chunk,split_first,split_later,split_backward[,emit_band] "
-                    "to set instead of autotuning (the 'tuning' line of an earlier run)")
+    ap.add_argument("--tune", default="", help="tile_batch,chunk,split_first,split_later,split_backward[,emit_band"
+                    "[,passes,growth]] to set instead of autotuning (the 'tuning' line of an earlier run)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0, help="only time a device copy of this many MiB")
     ap.add_argument("--valu-peak", action="store_true", help="only measure the sustained VALU issue rate")

@@ -25,8 +25,8 @@ EXPORTS = (
     "gsr_shape", "gsr_adam", "gsr_adam_sh", "gsr_stats", "gsr_batch_create", "gsr_batch_destroy", "gsr_batch_set_cameras",
     "gsr_batch_step", "gsr_batch_entries", "gsr_batch_configure", "gsr_batch_autotune", "gsr_batch_profile",
     "gsr_batch_set_passes", "gsr_loss_grad_window", "gsr_batch_set_ssim_window", "gsr_batch_set_split",
-    "gsr_batch_tuning", "gsr_batch_set_emit_band", "gsr_set_culling", "gsr_batch_step_stats", "gsr_batch_set_culling",
-    "gsr_batch_tile_lists", "gsr_bin_sort_noalloc",
+    "gsr_batch_tuning", "gsr_batch_passes", "gsr_batch_set_emit_band", "gsr_set_culling", "gsr_batch_step_stats",
+    "gsr_batch_set_culling", "gsr_batch_tile_lists", "gsr_bin_sort_noalloc",
 )
 SSIM_BOX7, SSIM_GAUSS11 = 7, 11
 STAGES = ("project", "depth_sort", "bin_count", "bin_emit", "forward", "loss_grad", "backward", "chain")
@@ -96,6 +96,7 @@ def load():
         "gsr_loss_grad_window": (i32, [P, ctypes.POINTER(Camera), P, P, f32, i32, P]),
         "gsr_batch_set_ssim_window": (i32, [P, i32]),
         "gsr_batch_tuning": (i32, [P, ctypes.POINTER(u32)]),
+        "gsr_batch_passes": (i32, [P, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "gsr_batch_set_emit_band": (i32, [P, u32]),
         "gsr_batch_set_split": (i32, [P, u32, u32, u32]),
         "gsr_set_culling": (i32, [P, i32]),
@@ -333,17 +334,23 @@ class Batch:
         self.ctx._check(self.lib.gsr_batch_set_split(self.h, forward_first, forward_later, backward))
 
     def tuning(self) -> tuple:
-        """(tile_batch, chunk, split_first, split_later, split_backward, emit_band) in effect."""
+        """(tile_batch, chunk, split_first, split_later, split_backward, emit_band,
+        depth-slice passes, growth) in effect."""
         out = (ctypes.c_uint32 * 6)()
         self.ctx._check(self.lib.gsr_batch_tuning(self.h, out))
-        return tuple(int(x) for x in out)
+        passes, growth = ctypes.c_uint32(), ctypes.c_uint32()
+        self.ctx._check(self.lib.gsr_batch_passes(self.h, ctypes.byref(passes), ctypes.byref(growth)))
+        return tuple(int(x) for x in out) + (passes.value, growth.value)
 
     def set_emit_band(self, tiles: int):
         self.ctx._check(self.lib.gsr_batch_set_emit_band(self.h, tiles))
 
     def set_tuning(self, tuning):
-        """Sets what tuning() returned (the emit band may be left off: 256)."""
+        """Sets what tuning() returned (the emit band may be left off: 256; the
+        depth slices too: the batch's own)."""
         tb, ch, sf, sl, sb = tuning[:5]
+        if len(tuning) > 6:
+            self.set_passes(tuning[6], tuning[7])
         self.configure(tb, ch)
         self.set_split(sf, sl, sb)
         self.set_emit_band(tuning[5] if len(tuning) > 5 else 256)

@@ -839,16 +839,17 @@ static napi_value js_batch_entries(napi_env env, napi_callback_info info) {
   return arr;
 }
 
-/* batchAutotune(batch, gs, sh, targets, lambda, grads, shape, shGrad) -> tuning (Uint32Array(6)) */
+/* batchAutotune(batch, gs, sh, targets, lambda, grads, shape, shGrad) -> tuning (Uint32Array(8)) */
 static napi_value js_batch_tuning_array(napi_env env, addon_batch* b) {
-  uint32_t t[6];
+  uint32_t t[8];
   int rc = gsr_batch_tuning(b->batch, t);
+  if (!rc) rc = gsr_batch_passes(b->batch, &t[6], &t[7]);
   if (rc) return throw_batch(env, b, rc);
   napi_value ab, arr;
   uint32_t* d;
   CHECK_NAPI(napi_create_arraybuffer(env, sizeof t, (void**)&d, &ab));
   memcpy(d, t, sizeof t);
-  CHECK_NAPI(napi_create_typedarray(env, napi_uint32_array, 6, ab, 0, &arr));
+  CHECK_NAPI(napi_create_typedarray(env, napi_uint32_array, 8, ab, 0, &arr));
   return arr;
 }
 
@@ -868,7 +869,8 @@ static napi_value js_batch_autotune(napi_env env, napi_callback_info info) {
   return js_batch_tuning_array(env, b);
 }
 
-/* batchTuning(batch) -> Uint32Array(6): tile batch, chunk, split first/later/backward, emit band */
+/* batchTuning(batch) -> Uint32Array(8): tile batch, chunk, split first/later/backward, emit band,
+ * depth-slice passes and growth */
 static napi_value js_batch_tuning(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
@@ -877,7 +879,7 @@ static napi_value js_batch_tuning(napi_env env, napi_callback_info info) {
   return js_batch_tuning_array(env, b);
 }
 
-/* batchSetTuning(batch, tuning[6]): the knobs batchTuning returned */
+/* batchSetTuning(batch, tuning[6 or 8]): the knobs batchTuning returned (the depth slices optional) */
 static napi_value js_batch_set_tuning(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return NULL;
@@ -890,7 +892,8 @@ static napi_value js_batch_set_tuning(napi_env env, napi_callback_info info) {
     return NULL;
   }
   const uint32_t* t = (const uint32_t*)data;
-  int rc = gsr_batch_configure(b->batch, t[0], t[1]);
+  int rc = len >= 8 * sizeof(uint32_t) ? gsr_batch_set_passes(b->batch, t[6], t[7]) : GSR_OK;
+  if (!rc) rc = gsr_batch_configure(b->batch, t[0], t[1]);
   if (!rc) rc = gsr_batch_set_split(b->batch, t[2], t[3], t[4]);
   if (!rc) rc = gsr_batch_set_emit_band(b->batch, t[5]);
   if (rc) return throw_batch(env, b, rc);

@@ -227,8 +227,9 @@ int gsr_batch_tile_lists(gsr_batch* batch, uint32_t view, uint32_t pass, uint32_
 /* Tuning knobs: tiles claimed per queue pop (0 = keep) and Gaussians per
  * binning chunk (0 = keep).  gsr_batch_autotune times candidates on the given
  * inputs and keeps the fastest; it writes the choice to *tile_batch / *chunk.
- * It also picks the split thresholds of gsr_batch_set_split.  Results do not
- * depend on any of these. */
+ * It also picks the depth slices of gsr_batch_set_passes (2 x 8 or 3 x 4, for
+ * n >= 65536), the emit band and the split thresholds of gsr_batch_set_split.
+ * Results do not depend on any of these. */
 int gsr_batch_configure(gsr_batch* batch, uint32_t tile_batch, uint32_t chunk);
 /* Walk lengths above which a tile is blended as two half-tile items: in the
  * forward's first depth slice (by the previous step's walk of the tile), in its
@@ -253,6 +254,8 @@ int gsr_batch_set_ssim_window(gsr_batch* batch, int window);
  * those of the full tile lists: a saturated tile's blend stops before any later
  * entry.  growth is ignored for passes == 1. */
 int gsr_batch_set_passes(gsr_batch* batch, uint32_t passes, uint32_t growth);
+/* The depth slices in effect (growth is 1 for one pass). */
+int gsr_batch_passes(gsr_batch* batch, uint32_t* passes, uint32_t* growth);
 /* Runs one step without the graph, timing each stage with HIP events on the
  * context's stream, then a second step that collects the counters.  stage_ms[GSR_STAGES] receives milliseconds for: project,
  * depth sort, bin count, bin emit, forward, loss gradient, backward, chain.

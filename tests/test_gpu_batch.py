@@ -148,18 +148,24 @@ def test_batch_tiny_gaussian_counts(ctx, n):
 
 
 def test_batch_tuning_reads_back_the_knobs(ctx):
-    """gsr_batch_tuning returns what configure / set_split set (0 keeps a knob)."""
+    """gsr_batch_tuning / gsr_batch_passes return what configure, set_split,
+    set_emit_band and set_passes set (0 keeps a knob); the depth slices of a
+    batch under 65536 Gaussians default to one pass."""
     sc = small_batch_scene(n=3000, views=2, seed=4)
     b = lib.Batch(ctx, sc.cams, sc.n, 3)
     b.set_tuning((4, 2048, 100, 200, 300, 128))
-    assert b.tuning() == (4, 2048, 100, 200, 300, 128)
+    assert b.tuning() == (4, 2048, 100, 200, 300, 128, 1, 1)
     b.configure(0, 0)
     b.set_split(0, 0, 0)
-    assert b.tuning() == (4, 2048, 0, 0, 0, 128)
+    assert b.tuning() == (4, 2048, 0, 0, 0, 128, 1, 1)
     b.set_tuning((1, 1024, 0, 0, 1024))  # the emit band left off: 256
-    assert b.tuning() == (1, 1024, 0, 0, 1024, 256)
+    assert b.tuning() == (1, 1024, 0, 0, 1024, 256, 1, 1)
+    b.set_tuning((1, 1024, 0, 0, 1024, 256, 3, 4))
+    assert b.tuning() == (1, 1024, 0, 0, 1024, 256, 3, 4)
     with pytest.raises(lib.GsrError):
         b.set_emit_band(32)
+    with pytest.raises(lib.GsrError):
+        b.set_passes(3, 64)  # growth above 16
     b.close()
 
 
