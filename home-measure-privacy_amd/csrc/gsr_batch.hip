@@ -57,6 +57,11 @@ struct gsr_batch {
   uint2* rects = nullptr;
   uint2* rect_sorted = nullptr;
   uint32_t *hist = nullptr, *piece = nullptr;
+  // [V][2]: the range of the depth keys' top 16 bits (three-pass depth sort).
+  uint32_t* key_range = nullptr;
+  // The depth sort takes four passes from the first step whose view had keys too
+  // wide for the three-pass sort (its order was not used: the step ran again).
+  bool wide_keys = false;
   uint32_t* matrix = nullptr;
   uint32_t* tile_tot = nullptr;
   // Depth-slice passes: pass p bins sorted positions [pass_end[p-1], pass_end[p]).
@@ -87,7 +92,8 @@ struct gsr_batch {
   uint64_t* valid_mask = nullptr;  // [V][ceil(n / 64)]: projected validly (STATS)
   uint32_t* order = nullptr;
   uint64_t order_cap = 0;
-  // [0] overflow, [16 + p] split-item count of forward pass p, [16 + kMaxPasses]
+  // [0] overflow (bit 0: entry storage, bit 1: a view's depth keys too wide for the
+  // three-pass sort), [16 + p] split-item count of forward pass p, [16 + kMaxPasses]
   // the backward's; [kLiveBase + p] live-list length of forward pass p,
   // [kLiveBase + kMaxPasses] the backward's; from kQueueBase, 9 work counters
   // per forward pass, then the backward's (a shared one and one per die part),
@@ -167,9 +173,10 @@ int alloc_all(gsr_batch* b) {
       (rc = dalloc(b, &b->keys, vn)) || (rc = dalloc(b, &b->keys_alt, vn)) || (rc = dalloc(b, &b->vals, vn)) ||
       (rc = dalloc(b, &b->vals_alt, vn)) || (rc = dalloc(b, &b->rects, vn)) ||
       (rc = dalloc(b, &b->rect_sorted, vn)) ||
-      (rc = dalloc(b, &b->hist, std::max<size_t>(1, radix_hist_elems(b->n, b->V)))) ||
+      (rc = dalloc(b, &b->hist, std::max<size_t>(1, radix_hist_elems(b->n, b->V, true)))) ||
       (rc = dalloc(b, &b->piece,
-                   std::max<size_t>(1, scan_piece_elems(radix_hist_elems(b->n, 1), b->V)))) ||
+                   std::max<size_t>(1, scan_piece_elems(radix_hist_elems(b->n, 1, true), b->V)))) ||
+      (rc = dalloc(b, &b->key_range, size_t(b->V) * 2)) ||
       (rc = dalloc(b, &b->tile_tot, b->total_tiles)) || (rc = dalloc(b, &b->entries, kMaxPasses * b->V)) ||
       (rc = dalloc(b, &b->ebase, kMaxPasses * b->V)) || (rc = dalloc(b, &b->run, 1)) ||
       (rc = dalloc(b, &b->ranges, size_t(kMaxPasses) * b->total_tiles * 2)) ||
@@ -253,7 +260,9 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
                  packed, !b->no_cull, b->valid_mask);
   mark();
   SortScratch sc{b->keys_alt, b->vals, b->vals_alt, b->hist, b->piece};
-  launch_depth_sort(s, b->keys, b->n, b->V, sc, b->rects, b->rect_sorted, packed);
+  if (!b->wide_keys) launch_zero(s, b->key_range, size_t(b->V) * 2 * sizeof(uint32_t));
+  launch_depth_sort(s, b->keys, b->n, b->V, sc, b->rects, b->rect_sorted, packed,
+                    b->wide_keys ? nullptr : b->key_range, b->flags);
   mark();
   RasterArgs a{};
   a.views = b->d_views;
@@ -419,7 +428,7 @@ int run_step(gsr_batch* b, const float* gs, const float* sh, const float* target
     if (rc) return rc;
     drop_graph(b);
   }
-  for (int attempt = 0; attempt < 3; ++attempt) {
+  for (int attempt = 0; attempt < 4; ++attempt) {
     if (use_graph) {
       std::vector<const void*> key = {gs, sh, targets, grads, shape, sh_grad, stats, images};
       if (!b->exec || key != b->graph_key || lambda != b->graph_lambda) {
@@ -450,14 +459,24 @@ int run_step(gsr_batch* b, const float* gs, const float* sh, const float* target
     }
     uint64_t total = 0;
     for (size_t k = 0; k < ne; ++k) total += b->pinned[k];
-    const bool overflow = (uint32_t(b->pinned[ne]) & 0xFFFFFFFFu) != 0 || total > b->order_cap;
-    if (!overflow) return GSR_OK;
-    // Grow the entry storage and run the step again.
-    uint64_t cap = total + total / 4 + 1024;
-    int rc = dalloc(b, &b->order, cap);
-    if (rc) return rc;
-    b->order_cap = cap;
-    drop_graph(b);
+    const uint32_t flag = uint32_t(b->pinned[ne]);
+    const bool overflow = (flag & 1u) != 0 || total > b->order_cap;
+    if (!overflow && !(flag & 2u)) return GSR_OK;
+    if (flag & 2u) {
+      // A view's depth keys were too wide for the three-pass sort, so its tile
+      // lists were not in depth order: four passes from now on.
+      b->wide_keys = true;
+      drop_graph(b);
+    }
+    if (overflow) {
+      // Grow the entry storage.
+      uint64_t cap = total + total / 4 + 1024;
+      int rc = dalloc(b, &b->order, cap);
+      if (rc) return rc;
+      b->order_cap = cap;
+      drop_graph(b);
+    }
+    // and run the step again.
   }
   return ctx_fail(b->ctx, GSR_E_CAPACITY, "entry storage kept overflowing");
 }

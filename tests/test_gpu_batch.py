@@ -262,3 +262,41 @@ def test_batch_entry_storage_grows(ctx):
         out = r.render(sc.gs, cam, sc.sh, 3)
         assert out.shape == (cam.height, cam.width, 3)
     r.close()
+
+
+def test_batch_wide_depth_range_sorts_with_four_passes(ctx):
+    """The batched depth sort takes three passes while each view's depth keys'
+    top 16 bits span at most 511 values (a depth range within a factor of about
+    16).  A scene with Gaussians from depth 1.5 to about 100 exceeds that: the
+    first step flags it, runs again with the four-pass sort, and renders as the
+    single-view path (bit-identical images) with the same gradients."""
+    sc = small_batch_scene(n=20000, views=3, seed=5)
+    rng = np.random.default_rng(5)
+    far = rng.choice(sc.n, 2000, replace=False)
+    d = rng.standard_normal((far.size, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    sc.gs[far, 0:3] = (d * rng.uniform(60.0, 100.0, (far.size, 1))).astype(np.float32)
+    sc.gs[far, 4:7] = np.log(1.5)  # a few pixels across at depth 80
+    sc.gs[far, 7] = 3.0
+    # The regime is exercised: some view's binned splats span more than 511
+    # top-16-bit key values (the square rectangle of tileLists bounds the hit box).
+    widest = 0
+    for cam in sc.cams:
+        sp = O.project(cam, sc.gs, sc.sh, 3, np.float32)
+        u, v, r = (sp[:, k].astype(np.float64) for k in (0, 1, 6))
+        on = (sp[:, 11] > 0.5) & (u + r > 0) & (u - r < cam.width) & (v + r > 0) & (v - r < cam.height)
+        top = sp[on, 7].view(np.uint32) >> 16
+        widest = max(widest, int(top.max()) - int(top.min()))
+    log(f"  widest view: {widest} top-16-bit key values")
+    assert widest > 511
+    outs, entries = run_batch(ctx, sc, graph=True, steps=2, images=True)
+    r = Renderer()
+    imgs, g_sum = [], np.zeros((sc.n, 16))
+    for cam, tgt in zip(sc.cams, sc.targets):
+        out = r.gradients(sc.gs, cam, tgt, 0.0, sh=sc.sh, deg=3)
+        imgs.append(out["image"].ravel())
+        g_sum += out["grads"]
+    r.close()
+    for g_b, _, _, img_b in outs:
+        np.testing.assert_array_equal(img_b, np.concatenate(imgs))
+        assert rel_l2(g_b[:, :4], g_sum[:, :4]) < 1e-5
