@@ -39,8 +39,16 @@ async function main() {
   }
   const data = await pending;
   addon.free(ctx, buf);  // accepted again once the read has settled
+  // The batch knobs read back as eight values, the depth slices last.
+  const camsBuf = new Uint8Array(meta.cams.length * 80);
+  meta.cams.forEach((c, v) => camsBuf.set(new Uint8Array(gpu.cameraUniform(c, 100000)), v * 80));
+  const bt = addon.batchCreate(ctx, camsBuf, 100000, 0);
+  const tuningDefault = Array.from(addon.batchTuning(bt));
+  addon.batchSetTuning(bt, new Uint32Array([1, 2048, 0, 0, 1024, 256, 3, 4]));
+  const tuningSet = Array.from(addon.batchTuning(bt));
+  addon.batchDestroy(bt);
   fs.writeFileSync(path.join(dir, 'done.json'), JSON.stringify({ ok: true, busy, readBytes: data.byteLength,
-    entries: out.entries }));
+    entries: out.entries, tuningDefault, tuningSet }));
 }
 
 main().catch((e) => {

@@ -26,7 +26,8 @@ def log(msg):
 def test_gradients_batch_matches_c_abi():
     """gradientsBatch (SH3, four views of uneven sizes) equals lib.Batch's step:
     images bit for bit, gradients and STATS to float-atomic summation order;
-    read() resolves, and a context with a read pending refuses other calls."""
+    read() resolves, and a context with a read pending refuses other calls;
+    batchTuning / batchSetTuning carry the eight knobs, depth slices included."""
     assert shutil.which("node"), "node missing"
     sc = scenes.config(3, n=5000, views=4, seed=21)
     sizes = [(96, 64), (160, 120), (64, 48), (33, 17)]
@@ -54,6 +55,9 @@ def test_gradients_batch_matches_c_abi():
               "shape": ld("shape.f32").reshape(-1, 10), "sh": ld("shgrad.f32").reshape(-1, 16, 3),
               "stats": ld("stats.f32").reshape(-1, 2)}
     assert done["busy"] and done["readBytes"] == 1 << 20
+    # batchTuning: six knobs, then the depth slices (2 x 8 by default at 100k Gaussians)
+    assert len(done["tuningDefault"]) == 8 and done["tuningDefault"][6:] == [2, 8]
+    assert done["tuningSet"] == [1, 2048, 0, 0, 1024, 256, 3, 4]
     n = sc.n
     with lib.Context(0) as ctx:
         gsb, shb = ctx.upload(sc.gs), ctx.upload(sc.sh)
