@@ -57,7 +57,8 @@ struct gsr_batch {
   uint2* rects = nullptr;
   uint2* rect_sorted = nullptr;
   uint32_t *hist = nullptr, *piece = nullptr;
-  // [V][2]: the range of the depth keys' top 16 bits (three-pass depth sort).
+  // The range of the depth keys' top 16 bits (three-pass depth sort): [V][2],
+  // then the chunks' [V][chunks][2].
   uint32_t* key_range = nullptr;
   // The depth sort takes four passes from the first step whose view had keys too
   // wide for the three-pass sort (its order was not used: the step ran again).
@@ -176,7 +177,7 @@ int alloc_all(gsr_batch* b) {
       (rc = dalloc(b, &b->hist, std::max<size_t>(1, radix_hist_elems(b->n, b->V, true)))) ||
       (rc = dalloc(b, &b->piece,
                    std::max<size_t>(1, scan_piece_elems(radix_hist_elems(b->n, 1, true), b->V)))) ||
-      (rc = dalloc(b, &b->key_range, size_t(b->V) * 2)) ||
+      (rc = dalloc(b, &b->key_range, depth_sort_range_elems(b->n, b->V))) ||
       (rc = dalloc(b, &b->tile_tot, b->total_tiles)) || (rc = dalloc(b, &b->entries, kMaxPasses * b->V)) ||
       (rc = dalloc(b, &b->ebase, kMaxPasses * b->V)) || (rc = dalloc(b, &b->run, 1)) ||
       (rc = dalloc(b, &b->ranges, size_t(kMaxPasses) * b->total_tiles * 2)) ||
@@ -260,7 +261,6 @@ void issue_step(gsr_batch* b, const float* gs, const float* sh, const float* tar
                  packed, !b->no_cull, b->valid_mask);
   mark();
   SortScratch sc{b->keys_alt, b->vals, b->vals_alt, b->hist, b->piece};
-  if (!b->wide_keys) launch_zero(s, b->key_range, size_t(b->V) * 2 * sizeof(uint32_t));
   launch_depth_sort(s, b->keys, b->n, b->V, sc, b->rects, b->rect_sorted, packed,
                     b->wide_keys ? nullptr : b->key_range, b->flags);
   mark();

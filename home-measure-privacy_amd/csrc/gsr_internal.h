@@ -86,13 +86,15 @@ struct SortScratch {
 // Histogram elements of the depth sort: 256 digits per chunk, or 512 (wide) for
 // the three-pass sort's offset pass.
 size_t radix_hist_elems(uint32_t n, uint32_t num_views, bool wide = false);
+// Elements of the three-pass sort's range scratch: [V][2], then [V][chunks][2].
+size_t depth_sort_range_elems(uint32_t n, uint32_t num_views);
 size_t scan_piece_elems(size_t seg_len, uint32_t num_segs);
 // Stable sort of (keys[v][i], i) per view by key; result indices in vals, and
 // rect_sorted[v][p] = rects[v][vals[v][p]] (the sorted keys themselves are not
-// kept: keys is scratch afterwards).  Four 8-bit passes; with range ([V][2],
-// zeroed before the launch; scratch sized wide) three, valid while each view's
-// keys' top 16 bits span at most 511 values: a wider view sets bit 1 of
-// *range_flag, and its order must not be used.
+// kept: keys is scratch afterwards).  Four 8-bit passes; with range (scratch of
+// depth_sort_range_elems; hist sized wide) three, valid while each view's keys'
+// top 16 bits span at most 511 values: a wider view sets bit 1 of *range_flag,
+// and its order must not be used.
 void launch_depth_sort(hipStream_t s, uint32_t* keys, uint32_t n, uint32_t num_views, const SortScratch& sc,
                        const uint2* rects, uint2* rect_sorted, bool packed, uint32_t* range = nullptr,
                        uint32_t* range_flag = nullptr);

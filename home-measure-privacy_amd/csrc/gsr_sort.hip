@@ -57,7 +57,8 @@ __device__ inline uint32_t key_digit(uint32_t key, int shift, uint32_t lo) {
 }
 
 // range[2 v] = 0xFFFF - the least top 16 bits of view v's valid keys, range[2 v +
-// 1] = the greatest (both start at 0 and grow by atomicMax).
+// 1] = the greatest (range_reduce_kernel; 0 and 0 for a view without valid keys).
+// From range + 2 V on: the same per (view, chunk), written by the first pass.
 __device__ inline uint32_t range_lo(const uint32_t* range, uint32_t v) { return 0xFFFFu - range[2 * v]; }
 
 // Digit histogram of one 4096-key chunk of one view: hist[v][digit][chunk].
@@ -65,10 +66,9 @@ __device__ inline uint32_t range_lo(const uint32_t* range, uint32_t v) { return 
 // key rarely collides; the top digit (sign and high exponent bits) takes a
 // handful of values, so there lanes with equal digits are matched by ballots
 // first and add their count once (kMatch).
-// kRange (the first pass of the three-pass sort) also folds the chunk's valid
-// keys into the view's range; kOffset (its last pass) flags a view whose range
-// exceeds the offset digit: range_flag |= 2, and the step is run again with the
-// four-pass sort.
+// kRange (the first pass of the three-pass sort) also stores the chunk's range
+// of its valid keys' top 16 bits (range_reduce_kernel then folds the chunks of
+// each view); kOffset (its last pass) digits by the view's range.
 template <int kBits, bool kMatch, bool kOffset, bool kRange>
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t* __restrict__ keys, uint32_t n,
                                                                    uint32_t chunks, int shift,
@@ -86,10 +86,6 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
   }
   __syncthreads();
   const uint32_t lo = kOffset ? range_lo(range, v) : 0u;
-  if (kOffset && c == 0 && threadIdx.x == 0) {
-    const uint32_t hi = range[2 * v + 1];
-    if (hi >= lo && hi - lo > 510u) atomicOr(range_flag, 2u);
-  }
   const size_t base = size_t(v) * n + size_t(c) * kSortChunk;
   const uint32_t len = min(uint32_t(kSortChunk), n - c * kSortChunk);
   // All of the thread's loads are issued before the first is used.
@@ -116,16 +112,60 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
     }
   }
   if (kRange) {
-    if (mn <= mx) {
+    // Over the wave first: every thread's LDS atomic on two words serialises.
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      mn = min(mn, uint32_t(__shfl_xor(int(mn), off)));
+      mx = max(mx, uint32_t(__shfl_xor(int(mx), off)));
+    }
+    if (__lane_id() == 0 && mn <= mx) {
       atomicMin(&rmin, mn);
       atomicMax(&rmax, mx);
     }
   }
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < kB; b += kSortThreads) hist[(size_t(v) * kB + b) * chunks + c] = h[b];
-  if (kRange && threadIdx.x == 0 && rmin <= rmax) {
-    atomicMax(range + 2 * v, 0xFFFFu - rmin);
-    atomicMax(range + 2 * v + 1, rmax);
+  if (kRange && threadIdx.x == 0) {
+    // Plain stores (no atomics: 245 blocks of a 1M-Gaussian view queued on one
+    // word took the pass from 53 to 122 us).
+    uint32_t* part = range + 2 * (size_t(gridDim.y) + size_t(v) * chunks + c);
+    part[0] = 0xFFFFu - rmin;
+    part[1] = rmax;
+  }
+}
+
+// One block per view folds the chunks' ranges into the view's, and flags a view
+// whose valid keys' top 16 bits span more than 511 values: range_flag |= 2, its
+// three-pass order is not valid and the step is run again with four passes.
+__global__ __launch_bounds__(kSortThreads) void range_reduce_kernel(uint32_t* __restrict__ range, uint32_t chunks,
+                                                                   uint32_t* __restrict__ range_flag) {
+  __shared__ uint32_t wlo[kSortThreads / 64], whi[kSortThreads / 64];
+  const uint32_t v = blockIdx.x;
+  const uint32_t* part = range + 2 * (size_t(gridDim.x) + size_t(v) * chunks);
+  uint32_t ilo = 0u, hi = 0u;  // ilo: 0xFFFF - the least (the larger, the less)
+  for (uint32_t c = threadIdx.x; c < chunks; c += kSortThreads) {
+    ilo = max(ilo, part[2 * c]);
+    hi = max(hi, part[2 * c + 1]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    ilo = max(ilo, uint32_t(__shfl_xor(int(ilo), off)));
+    hi = max(hi, uint32_t(__shfl_xor(int(hi), off)));
+  }
+  if (__lane_id() == 0) {
+    wlo[threadIdx.x / 64] = ilo;
+    whi[threadIdx.x / 64] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kSortThreads / 64; ++w) {
+      ilo = max(ilo, wlo[w]);
+      hi = max(hi, whi[w]);
+    }
+    range[2 * v] = ilo;
+    range[2 * v + 1] = hi;
+    const uint32_t lo = 0xFFFFu - ilo;
+    if (hi >= lo && hi - lo > 510u) atomicOr(range_flag, 2u);
   }
 }
 
@@ -794,6 +834,11 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
 
 }  // namespace
 
+size_t depth_sort_range_elems(uint32_t n, uint32_t num_views) {
+  const size_t chunks = (size_t(n) + kSortChunk - 1) / kSortChunk;
+  return 2 * size_t(num_views) * (1 + chunks);
+}
+
 size_t radix_hist_elems(uint32_t n, uint32_t num_views, bool wide) {
   size_t chunks = (size_t(n) + kSortChunk - 1) / kSortChunk;
   return size_t(num_views) * (wide ? (1u << kWideBits) : kBins) * chunks;
@@ -808,7 +853,7 @@ void launch_depth_sort(hipStream_t s, uint32_t* keys, uint32_t n, uint32_t num_v
   if (n == 0 || num_views == 0) return;
   const uint32_t chunks = (n + kSortChunk - 1) / kSortChunk;
   dim3 grid(chunks, num_views);
-  // With range (zeroed by the caller): three passes, bits 0-7, bits 8-15, then
+  // With range ([V][2] then [V][chunks][2]): three passes, bits 0-7, bits 8-15, then
   // the 9-bit offset of the top 16 bits, for views whose valid keys' top 16 bits
   // span at most 511 values (a depth range within a factor of about 16; 292 on
   // the 32-view bench scene).  A wider view sets range_flag bit 1 and its order
@@ -840,6 +885,8 @@ void launch_depth_sort(hipStream_t s, uint32_t* keys, uint32_t n, uint32_t num_v
       hipLaunchKernelGGL((radix_hist_kernel<kRadixBits, false, false, false>), grid, dim3(kSortThreads), 0, s, kin, n,
                          chunks, shift, sc.hist, range, range_flag);
     scan_segments(s, sc.hist, bins * chunks, num_views, sc.piece);
+    if (range && pass == 0)
+      hipLaunchKernelGGL(range_reduce_kernel, dim3(num_views), dim3(kSortThreads), 0, s, range, chunks, range_flag);
     if (offset)
       hipLaunchKernelGGL((radix_scatter_kernel<kWideBits, true>), grid, dim3(kSortThreads), 0, s, kin, vin, n, chunks,
                          shift, sc.hist, kout, vout, rects, rd, packed, range);
