@@ -116,7 +116,28 @@ struct BinPass {
   uint64_t* ebase;      // [V] first order slot of each view's entries in this pass
   uint64_t* entries;    // [V] entries per view in this pass
   bool first;           // first pass of the step: order slots start at 0
+  // A filtered pass (launch_slice_filter): the kept Gaussians of the slice, in
+  // depth order, rects[v][0 .. kept[v]) and vals[v][0 .. kept[v]) (view stride n),
+  // read instead of the sorted arrays' [pbeg, pend).  Null: not filtered.
+  const uint2* rects = nullptr;
+  const uint32_t* vals = nullptr;
+  const uint32_t* kept = nullptr;
 };
+// Before a later slice is binned: keeps, in depth order, the Gaussians of sorted
+// positions [pbeg, pend) whose rectangle meets a tile that done leaves open,
+// into rects / vals (view stride n), kept[v] of them per view.  Only for views
+// of at most slice_filter_fits' tiles.
+struct SliceFilter {
+  uint16_t* sat;   // [total tiles]: summed-area table of the open tiles
+  uint64_t* mask;  // [V][ceil(n / 64)]: kept bits
+  uint32_t* piece; // [V][slice_filter_pieces(n)]
+};
+bool slice_filter_fits(uint32_t max_tiles);
+size_t slice_filter_pieces(uint32_t n);
+void launch_slice_filter(hipStream_t s, const ViewDesc* views, uint32_t num_views, uint32_t max_tiles, uint32_t n,
+                         uint32_t pbeg, uint32_t pend, const uint2* rect_sorted, const uint32_t* sorted_vals,
+                         bool packed, const uint8_t* done, const SliceFilter& f, uint2* rects, uint32_t* vals,
+                         uint32_t* kept);
 // Pass part 1: per-(chunk, tile) counts, tile totals, ranges, entry counts and
 // entry bases.  host_views: the views on the host (grid sizing).
 void launch_bin_count(hipStream_t s, const ViewDesc* views, const ViewDesc* host_views, uint32_t num_views,

@@ -425,6 +425,13 @@ __device__ inline Band band_of(const ViewDesc& vd, uint32_t band, uint32_t budge
   return b;
 }
 
+// Chunks of view v that hold positions of the pass: all of them, or for a
+// filtered pass (kept set) those below the view's kept count.  Counts of the
+// chunks past it are neither written nor read.
+__device__ inline uint32_t live_chunks(uint32_t v, uint32_t chunks, uint32_t chunk, const uint32_t* kept) {
+  return kept ? min(chunks, (kept[v] + chunk - 1) / chunk) : chunks;
+}
+
 // Pass 1: per (chunk, tile) entry counts.  Each Gaussian adds its rectangle
 // (clipped to the band) to a 2D difference array with four LDS atomics; prefix
 // sums along x then y turn it into counts.  Done tiles count zero.
@@ -433,17 +440,18 @@ __global__ __launch_bounds__(64) void bin_count_kernel(const ViewDesc* __restric
                                                        uint32_t pbeg, uint32_t pend, uint32_t chunks,
                                                        const uint2* __restrict__ sorted,
                                                        const uint8_t* __restrict__ done,
-                                                       uint32_t* __restrict__ matrix) {
+                                                       uint32_t* __restrict__ matrix,
+                                                       const uint32_t* __restrict__ kept) {
   extern __shared__ int diff[];
   const uint32_t v = blockIdx.z, c = blockIdx.x, lane = threadIdx.x;
   const ViewDesc vd = views[v];
   const Band bd = band_of(vd, blockIdx.y, kBandTiles);
-  if (!bd.valid) return;
+  if (!bd.valid || c >= live_chunks(v, chunks, chunk, kept)) return;
   const uint32_t rows = bd.r1 - bd.r0, tx = bd.tx, W = tx + 1;
   for (uint32_t k = lane; k < (rows + 1) * W; k += 64) diff[k] = 0;
   __syncthreads();
   const size_t seg = size_t(v) * n;
-  const uint32_t begin = pbeg + c * chunk, end = min(pend, begin + chunk);
+  const uint32_t begin = pbeg + c * chunk, end = min(kept ? kept[v] : pend, begin + chunk);
   // Eight rectangles per lane are loaded before any is used (past the end: the
   // empty rectangle).
   constexpr uint32_t kAhead = 8;
@@ -505,23 +513,25 @@ __global__ __launch_bounds__(64) void bin_count_kernel(const ViewDesc* __restric
 // tile_tot[tile_base + t] = sum over chunks of the pass's counts.
 __global__ __launch_bounds__(256) void bin_colsum_kernel(const ViewDesc* __restrict__ views, uint32_t chunks,
                                                          const uint32_t* __restrict__ matrix,
-                                                         uint32_t* __restrict__ tile_tot) {
+                                                         uint32_t* __restrict__ tile_tot, uint32_t chunk,
+                                                         const uint32_t* __restrict__ kept) {
   const uint32_t v = blockIdx.y;
   const ViewDesc vd = views[v];
   const uint32_t T = vd.tiles_x * vd.tiles_y;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T) return;
   const uint32_t* m = matrix + size_t(chunks) * vd.tile_base + t;
+  const uint32_t live = live_chunks(v, chunks, chunk, kept);
   // Eight rows in flight per thread: one at a time leaves the walk latency-bound.
   uint32_t s = 0, c = 0;
-  for (; c + 8 <= chunks; c += 8) {
+  for (; c + 8 <= live; c += 8) {
     uint32_t x[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) x[k] = m[size_t(c + k) * T];
 #pragma unroll
     for (int k = 0; k < 8; ++k) s += x[k];
   }
-  for (; c < chunks; ++c) s += m[size_t(c) * T];
+  for (; c < live; ++c) s += m[size_t(c) * T];
   tile_tot[vd.tile_base + t] = s;
 }
 
@@ -566,7 +576,8 @@ __global__ void bin_entry_base_kernel(uint32_t num_views, const uint64_t* __rest
 // matrix[v][c][t] <- ranges start of t + sum_{c' < c} matrix[v][c'][t].
 __global__ __launch_bounds__(256) void bin_colscan_kernel(const ViewDesc* __restrict__ views, uint32_t chunks,
                                                           const uint32_t* __restrict__ ranges,
-                                                          uint32_t* __restrict__ matrix) {
+                                                          uint32_t* __restrict__ matrix, uint32_t chunk,
+                                                          const uint32_t* __restrict__ kept) {
   const uint32_t v = blockIdx.y;
   const ViewDesc vd = views[v];
   const uint32_t T = vd.tiles_x * vd.tiles_y;
@@ -574,8 +585,9 @@ __global__ __launch_bounds__(256) void bin_colscan_kernel(const ViewDesc* __rest
   if (t >= T) return;
   uint32_t* m = matrix + size_t(chunks) * vd.tile_base + t;
   uint32_t run = ranges[size_t(vd.tile_base + t) * 2];
+  const uint32_t live = live_chunks(v, chunks, chunk, kept);
   uint32_t c = 0;
-  for (; c + 8 <= chunks; c += 8) {  // eight rows loaded before any is rewritten
+  for (; c + 8 <= live; c += 8) {  // eight rows loaded before any is rewritten
     uint32_t x[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) x[k] = m[size_t(c + k) * T];
@@ -585,7 +597,7 @@ __global__ __launch_bounds__(256) void bin_colscan_kernel(const ViewDesc* __rest
       run += x[k];
     }
   }
-  for (; c < chunks; ++c) {
+  for (; c < live; ++c) {
     const uint32_t x = m[size_t(c) * T];
     m[size_t(c) * T] = run;
     run += x;
@@ -651,14 +663,15 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
                                                       const uint32_t* __restrict__ matrix,
                                                       const uint64_t* __restrict__ ebase_v,
                                                       uint32_t* __restrict__ order, uint64_t capacity,
-                                                      uint32_t* __restrict__ overflow, uint32_t emit_band) {
+                                                      uint32_t* __restrict__ overflow, uint32_t emit_band,
+                                                      const uint32_t* __restrict__ kept) {
   extern __shared__ uint4 lds_quads[];
   const uint32_t X = gridDim.x, XY = gridDim.x * gridDim.y;
   const uint32_t logical = emit_block(blockIdx.x + X * blockIdx.y + XY * blockIdx.z, XY * gridDim.z);
   const uint32_t v = logical / XY, c = logical % X, lane = threadIdx.x;
   const ViewDesc vd = views[v];
   const Band bd = band_of(vd, (logical % XY) / X, emit_band);
-  if (!bd.valid) return;
+  if (!bd.valid || c >= live_chunks(v, chunks, chunk, kept)) return;
   const uint32_t tx = bd.tx, ntile = (bd.r1 - bd.r0) * tx;
   // queue record: band-local tile of the rectangle's corner, width, pair count
   // (replaced by the first pair's position during a flush), Gaussian index
@@ -702,7 +715,7 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
   if (__ballot(open) == 0) return;  // every tile of the band is saturated
   __syncthreads();
   const size_t seg = size_t(v) * n;
-  const uint32_t begin = pbeg + c * chunk, end = min(pend, begin + chunk);
+  const uint32_t begin = pbeg + c * chunk, end = min(kept ? kept[v] : pend, begin + chunk);
   const uint64_t ebase = ebase_v[v];
   const uint64_t below = (uint64_t(1) << lane) - 1;
   bool over = false;
@@ -832,6 +845,150 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
   if (over) atomicOr(overflow, 1u);
 }
 
+// ---- later depth slices: the Gaussians that reach an open tile -------------------
+// A later slice is binned only for the tiles the slices before it left open
+// (unsaturated), and most of its Gaussians reach none of them: the count and emit
+// kernels would scan every one of them for every band with an open tile.  A
+// stable filter first keeps, in depth order, the Gaussians whose rectangle meets
+// an open tile.  The lists do not change: a Gaussian whose rectangle meets no
+// open tile adds no entry to the pass.
+constexpr uint32_t kFilterPiece = 4096;  // sorted positions per wave of the filter
+constexpr uint32_t kFilterWaves = 4;     // waves (pieces) per block
+constexpr uint32_t kMaxFilterTiles = 32768;  // views filtered: 64 KB of LDS table
+
+// sat[tile_base + y tx + x] = open tiles (done[.] == 0) in columns 0..x of rows
+// 0..y of the view, modulo 2^16: a summed-area table, so a rectangle's open tiles
+// take four reads.  The four-corner sum is exact modulo 2^16, and a view of at
+// most kMaxFilterTiles tiles holds fewer than 2^16 tiles in any rectangle, so
+// the sum is zero only when the rectangle holds no open tile.  One block per
+// view: rows by wave scans, then columns down.
+__global__ __launch_bounds__(1024) void open_sat_kernel(const ViewDesc* __restrict__ views,
+                                                        const uint8_t* __restrict__ done, uint16_t* __restrict__ sat) {
+  const ViewDesc vd = views[blockIdx.x];
+  const uint32_t tx = vd.tiles_x, ty = vd.tiles_y, lane = __lane_id(), w = threadIdx.x >> 6;
+  uint16_t* st = sat + vd.tile_base;
+  const uint8_t* dn = done + vd.tile_base;
+  for (uint32_t y = w; y < ty; y += 16) {  // wave-uniform loops: every lane takes part in the scans
+    uint32_t carry = 0;
+    for (uint32_t x0 = 0; x0 < tx; x0 += 64) {
+      const uint32_t x = x0 + lane;
+      const uint32_t open = x < tx && dn[size_t(y) * tx + x] == 0 ? 1u : 0u;
+      const uint32_t inc = wave_inclusive_scan<false>(open) + carry;
+      if (x < tx) st[size_t(y) * tx + x] = uint16_t(inc);
+      carry = __builtin_amdgcn_readlane(inc, 63);
+    }
+  }
+  __syncthreads();
+  for (uint32_t x = threadIdx.x; x < tx; x += 1024) {
+    uint32_t run = 0, y = 0;
+    for (; y + 8 <= ty; y += 8) {  // eight rows loaded before any is rewritten
+      uint32_t r[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) r[k] = st[size_t(y + k) * tx + x];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        run += r[k];
+        st[size_t(y + k) * tx + x] = uint16_t(run);
+      }
+    }
+    for (; y < ty; ++y) {
+      run += st[size_t(y) * tx + x];
+      st[size_t(y) * tx + x] = uint16_t(run);
+    }
+  }
+}
+
+// True when the rectangle (inclusive, within the view's tiles, or empty) holds an
+// open tile; st is the view's table in LDS.
+__device__ inline bool meets_open(const uint16_t* st, uint32_t tx, const Rect& r) {
+  if (r.x0 > r.x1 || r.y0 > r.y1) return false;
+  const uint32_t a = st[r.y1 * tx + r.x1];
+  const uint32_t b = r.x0 ? st[r.y1 * tx + r.x0 - 1] : 0u;
+  const uint32_t c = r.y0 ? st[(r.y0 - 1) * tx + r.x1] : 0u;
+  const uint32_t d = r.x0 && r.y0 ? st[(r.y0 - 1) * tx + r.x0 - 1] : 0u;
+  return ((a - b - c + d) & 0xFFFFu) != 0u;
+}
+
+// Wave pc of a view's slice [pbeg, pend) tests positions [pbeg + pc kFilterPiece,
+// +kFilterPiece): mask[v][(p - pbeg) / 64] gets the kept bits of each 64 of them,
+// piece[v][pc] the wave's kept count.
+template <bool kPacked>
+__global__ __launch_bounds__(64 * kFilterWaves) void slice_mark_kernel(
+    const ViewDesc* __restrict__ views, uint32_t n, uint32_t pbeg, uint32_t pend, uint32_t pieces,
+    const uint2* __restrict__ sorted, const uint16_t* __restrict__ sat, uint64_t* __restrict__ mask,
+    uint32_t mask_words, uint32_t* __restrict__ piece) {
+  extern __shared__ uint16_t lsat[];
+  const uint32_t v = blockIdx.y, lane = __lane_id(), pc = blockIdx.x * kFilterWaves + (threadIdx.x >> 6);
+  const ViewDesc vd = views[v];
+  const uint32_t T = vd.tiles_x * vd.tiles_y;
+  for (uint32_t k = threadIdx.x; k < T; k += 64 * kFilterWaves) lsat[k] = sat[vd.tile_base + k];
+  __syncthreads();
+  if (pc >= pieces) return;
+  const size_t seg = size_t(v) * n;
+  const uint32_t begin = pbeg + pc * kFilterPiece, end = min(pend, begin + kFilterPiece);
+  uint64_t* mk = mask + size_t(v) * mask_words;
+  uint32_t count = 0;
+  constexpr uint32_t kAhead = 8;
+  for (uint32_t p0 = begin; p0 < end; p0 += 64 * kAhead) {
+    uint2 rc[kAhead];
+#pragma unroll
+    for (uint32_t u = 0; u < kAhead; ++u) {
+      const uint32_t p = p0 + 64 * u + lane;
+      rc[u] = p < end ? load_rect<kPacked>(sorted, seg + p) : empty_rect();
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kAhead; ++u) {
+      const uint64_t m = __ballot(meets_open(lsat, vd.tiles_x, unpack_rect<kPacked>(rc[u])));
+      const uint32_t g = p0 + 64 * u;
+      if (g < end && lane == 0) mk[(g - pbeg) / 64] = m;
+      count += uint32_t(__popcll(m));
+    }
+  }
+  if (lane == 0) piece[size_t(v) * pieces + pc] = count;
+}
+
+// After the pieces are scanned per view: wave pc copies its kept Gaussians'
+// rectangles and indices to out_rects / out_vals[v][piece base + rank], in
+// position order (only the kept ones are loaded), and the view's last piece
+// writes kept[v].
+template <bool kPacked>
+__global__ __launch_bounds__(64 * kFilterWaves) void slice_keep_kernel(
+    uint32_t n, uint32_t pbeg, uint32_t pend, uint32_t pieces, const uint2* __restrict__ sorted,
+    const uint32_t* __restrict__ vals, const uint64_t* __restrict__ mask, uint32_t mask_words,
+    const uint32_t* __restrict__ piece, uint2* __restrict__ out_rects, uint32_t* __restrict__ out_vals,
+    uint32_t* __restrict__ kept) {
+  const uint32_t v = blockIdx.y, lane = __lane_id(), pc = blockIdx.x * kFilterWaves + (threadIdx.x >> 6);
+  if (pc >= pieces) return;
+  const size_t seg = size_t(v) * n;
+  const uint32_t begin = pbeg + pc * kFilterPiece, end = min(pend, begin + kFilterPiece);
+  const uint64_t* mk = mask + size_t(v) * mask_words;
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  uint32_t run = piece[size_t(v) * pieces + pc];  // wave-uniform
+  constexpr uint32_t kAhead = 8;
+  for (uint32_t p0 = begin; p0 < end; p0 += 64 * kAhead) {
+    uint64_t m[kAhead];
+#pragma unroll
+    for (uint32_t u = 0; u < kAhead; ++u) {
+      const uint32_t g = p0 + 64 * u;
+      m[u] = g < end ? mk[(g - pbeg) / 64] : 0ull;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kAhead; ++u) {
+      if ((m[u] >> lane) & 1u) {  // kept bits lie below end
+        const size_t p = seg + p0 + 64 * u + lane;
+        const size_t dst = seg + run + uint32_t(__popcll(m[u] & below));
+        out_vals[dst] = vals[p];
+        if (kPacked)
+          reinterpret_cast<uint32_t*>(out_rects)[dst] = reinterpret_cast<const uint32_t*>(sorted)[p];
+        else
+          out_rects[dst] = sorted[p];
+      }
+      run += uint32_t(__popcll(m[u]));
+    }
+  }
+  if (lane == 0 && pc == pieces - 1) kept[v] = run;
+}
+
 }  // namespace
 
 size_t depth_sort_range_elems(uint32_t n, uint32_t num_views) {
@@ -938,14 +1095,18 @@ void launch_bin_count(hipStream_t s, const ViewDesc* views, const ViewDesc* host
   const BandGeometry g = band_geometry(host_views, num_views, bs.emit_band);
   if (chunks > 0) {
     const dim3 grid(chunks, g.max_bands, num_views);
+    // A filtered pass reads its kept Gaussians from position 0.
+    const uint2* rs = pass.kept ? pass.rects : bs.rect_sorted;
+    const uint32_t pb = pass.kept ? 0u : pass.pbeg, pe = pass.kept ? pass.pend - pass.pbeg : pass.pend;
     if (bs.packed)
-      hipLaunchKernelGGL(bin_count_kernel<true>, grid, dim3(64), g.count_lds, s, views, n, chunk, pass.pbeg,
-                         pass.pend, chunks, bs.rect_sorted, pass.done, bs.matrix);
+      hipLaunchKernelGGL(bin_count_kernel<true>, grid, dim3(64), g.count_lds, s, views, n, chunk, pb, pe, chunks, rs,
+                         pass.done, bs.matrix, pass.kept);
     else
-      hipLaunchKernelGGL(bin_count_kernel<false>, grid, dim3(64), g.count_lds, s, views, n, chunk, pass.pbeg,
-                         pass.pend, chunks, bs.rect_sorted, pass.done, bs.matrix);
+      hipLaunchKernelGGL(bin_count_kernel<false>, grid, dim3(64), g.count_lds, s, views, n, chunk, pb, pe, chunks,
+                         rs, pass.done, bs.matrix, pass.kept);
     dim3 tgrid((g.max_tiles + 255) / 256, num_views);
-    hipLaunchKernelGGL(bin_colsum_kernel, tgrid, dim3(256), 0, s, views, chunks, bs.matrix, bs.tile_tot);
+    hipLaunchKernelGGL(bin_colsum_kernel, tgrid, dim3(256), 0, s, views, chunks, bs.matrix, bs.tile_tot, chunk,
+                       pass.kept);
   }
   hipLaunchKernelGGL(bin_tilescan_kernel, dim3(num_views), dim3(1024), 0, s, views, chunks, bs.tile_tot, pass.ranges,
                      pass.entries);
@@ -959,22 +1120,62 @@ void launch_bin_emit(hipStream_t s, const ViewDesc* views, const ViewDesc* host_
   const uint32_t chunks = slice_chunks(pass, chunk);
   if (num_views == 0 || chunks == 0) return;
   // Later slices bin only unsaturated tiles (a third of them on the 32-view
-  // bench scene): their bands are four times taller, so each chunk's
-  // rectangles are scanned by a quarter as many blocks for about the same
-  // pairs (emit 1.90 -> 1.53 ms there; twice and eight times were slower).
-  const uint32_t band = pass.done ? std::min(bs.emit_band * kLaterBandScale, kMaxEmitBand) : bs.emit_band;
+  // bench scene).  Unfiltered, their bands are four times taller, so each
+  // chunk's rectangles are scanned by a quarter as many blocks for about the
+  // same pairs (emit 1.90 -> 1.53 ms there; twice and eight times were slower).
+  // Filtered, a chunk holds only Gaussians with pairs, so the scans are cheap
+  // and the regular bands give four times the blocks (later-slice emit 0.62 ->
+  // 0.38 ms on the bench scene, 2.46 -> 1.89 ms on the 3M-Gaussian training
+  // scene; twice as tall: 0.45 ms).
+  const uint32_t band =
+      pass.done && !pass.kept ? std::min(bs.emit_band * kLaterBandScale, kMaxEmitBand) : bs.emit_band;
   const BandGeometry g = band_geometry(host_views, num_views, band);
   dim3 tgrid((g.max_tiles + 255) / 256, num_views);
-  hipLaunchKernelGGL(bin_colscan_kernel, tgrid, dim3(256), 0, s, views, chunks, pass.ranges, bs.matrix);
+  hipLaunchKernelGGL(bin_colscan_kernel, tgrid, dim3(256), 0, s, views, chunks, pass.ranges, bs.matrix, chunk,
+                     pass.kept);
   const dim3 grid(chunks, g.max_emit_bands, num_views);
+  const uint2* rs = pass.kept ? pass.rects : bs.rect_sorted;
+  const uint32_t* vs = pass.kept ? pass.vals : vals;
+  const uint32_t pb = pass.kept ? 0u : pass.pbeg, pe = pass.kept ? pass.pend - pass.pbeg : pass.pend;
   if (bs.packed)
-    hipLaunchKernelGGL(bin_emit_kernel<true>, grid, dim3(64), g.emit_lds, s, views, n, chunk, pass.pbeg, pass.pend,
-                       chunks, vals, bs.rect_sorted, pass.done, bs.matrix, pass.ebase, order, order_capacity,
-                       overflow, band);
+    hipLaunchKernelGGL(bin_emit_kernel<true>, grid, dim3(64), g.emit_lds, s, views, n, chunk, pb, pe, chunks, vs, rs,
+                       pass.done, bs.matrix, pass.ebase, order, order_capacity, overflow, band, pass.kept);
   else
-    hipLaunchKernelGGL(bin_emit_kernel<false>, grid, dim3(64), g.emit_lds, s, views, n, chunk, pass.pbeg, pass.pend,
-                       chunks, vals, bs.rect_sorted, pass.done, bs.matrix, pass.ebase, order, order_capacity,
-                       overflow, band);
+    hipLaunchKernelGGL(bin_emit_kernel<false>, grid, dim3(64), g.emit_lds, s, views, n, chunk, pb, pe, chunks, vs,
+                       rs, pass.done, bs.matrix, pass.ebase, order, order_capacity, overflow, band, pass.kept);
+}
+
+bool slice_filter_fits(uint32_t max_tiles) { return max_tiles <= kMaxFilterTiles; }
+
+size_t slice_filter_pieces(uint32_t n) { return (size_t(n) + kFilterPiece - 1) / kFilterPiece; }
+
+void launch_slice_filter(hipStream_t s, const ViewDesc* views, uint32_t num_views, uint32_t max_tiles, uint32_t n,
+                         uint32_t pbeg, uint32_t pend, const uint2* rect_sorted, const uint32_t* sorted_vals,
+                         bool packed, const uint8_t* done, const SliceFilter& f, uint2* rects, uint32_t* vals,
+                         uint32_t* kept) {
+  if (num_views == 0) return;
+  if (pend <= pbeg) {
+    launch_zero(s, kept, size_t(num_views) * sizeof(uint32_t));
+    return;
+  }
+  const uint32_t pieces = (pend - pbeg + kFilterPiece - 1) / kFilterPiece;
+  const uint32_t mask_words = (n + 63) / 64;
+  const dim3 grid((pieces + kFilterWaves - 1) / kFilterWaves, num_views), block(64 * kFilterWaves);
+  const size_t lds = (size_t(max_tiles) * sizeof(uint16_t) + 15) / 16 * 16;
+  hipLaunchKernelGGL(open_sat_kernel, dim3(num_views), dim3(1024), 0, s, views, done, f.sat);
+  if (packed)
+    hipLaunchKernelGGL(slice_mark_kernel<true>, grid, block, lds, s, views, n, pbeg, pend, pieces, rect_sorted, f.sat,
+                       f.mask, mask_words, f.piece);
+  else
+    hipLaunchKernelGGL(slice_mark_kernel<false>, grid, block, lds, s, views, n, pbeg, pend, pieces, rect_sorted,
+                       f.sat, f.mask, mask_words, f.piece);
+  hipLaunchKernelGGL(scan_pieces_kernel, dim3(num_views), dim3(1024), 0, s, f.piece, pieces);
+  if (packed)
+    hipLaunchKernelGGL(slice_keep_kernel<true>, grid, block, 0, s, n, pbeg, pend, pieces, rect_sorted, sorted_vals,
+                       f.mask, mask_words, f.piece, rects, vals, kept);
+  else
+    hipLaunchKernelGGL(slice_keep_kernel<false>, grid, block, 0, s, n, pbeg, pend, pieces, rect_sorted, sorted_vals,
+                       f.mask, mask_words, f.piece, rects, vals, kept);
 }
 
 }  // namespace gsr

@@ -227,8 +227,9 @@ int gsr_batch_tile_lists(gsr_batch* batch, uint32_t view, uint32_t pass, uint32_
 /* Tuning knobs: tiles claimed per queue pop (0 = keep) and Gaussians per
  * binning chunk (0 = keep).  gsr_batch_autotune times candidates on the given
  * inputs and keeps the fastest; it writes the choice to *tile_batch / *chunk.
- * It also picks the depth slices of gsr_batch_set_passes (2 x 8 or 3 x 4, for
- * n >= 65536), the emit band and the split thresholds of gsr_batch_set_split.
+ * It also picks the depth slices of gsr_batch_set_passes (2 x 8, 3 x 4 or
+ * 4 x 4, for n >= 65536), the emit band and the split thresholds of
+ * gsr_batch_set_split.
  * Results do not depend on any of these. */
 int gsr_batch_configure(gsr_batch* batch, uint32_t tile_batch, uint32_t chunk);
 /* Walk lengths above which a tile is blended as two half-tile items: in the
