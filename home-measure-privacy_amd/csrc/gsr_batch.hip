@@ -863,46 +863,50 @@ int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const flo
                    [&](size_t i) { return gsr_batch_set_passes(b, slices[i].first, slices[i].second); }, k)))
       return rc;
   }
-  // Tile batch x binning chunk.
-  std::vector<std::pair<uint32_t, uint32_t>> grid = {{b->tile_batch, b->chunk}};
-  for (uint32_t ch : {1024u, 2048u, 4096u, 8192u})
-    for (uint32_t tb : {1u, 2u, 4u, 8u})
-      if (tb != b->tile_batch || ch != b->chunk) grid.push_back({tb, ch});
-  if ((rc = pick(grid.size(), [&](size_t i) { return gsr_batch_configure(b, grid[i].first, grid[i].second); }, k)))
-    return rc;
-  // The emit band: small views keep more waves resident with 128 tiles, large
-  // ones rescan their chunks less with 256.
-  std::vector<uint32_t> bands = {b->emit_band};
-  for (uint32_t eb : {128u, 256u})
-    if (eb != b->emit_band) bands.push_back(eb);
-  if ((rc = pick(bands.size(), [&](size_t i) { return gsr_batch_set_emit_band(b, bands[i]); }, k))) return rc;
-  // The walk above which the backward blends a tile as two half-tile items: low
-  // enough to shorten the launch's tail, high enough not to repeat each entry's
-  // staging for tiles that are merely long (scenes of dense clusters prefer a
-  // higher one).
-  auto values = [](uint32_t current, std::initializer_list<uint32_t> cands) {
-    std::vector<uint32_t> v = {current};
-    for (uint32_t c : cands)
-      if (c != current) v.push_back(c);
-    return v;
-  };
-  const std::vector<uint32_t> backs = values(b->split_backward, {512u, 1024u, 2048u, 4096u});
-  if ((rc = pick(backs.size(),
-                 [&](size_t i) { return gsr_batch_set_split(b, b->split_first, b->split_later, backs[i]); }, k)))
-    return rc;
-  // Whether (and from which list length) the forward's later slices split their
-  // tiles: their unsaturated tiles walk whole segments, so a few long lists can
-  // end the launch late; splitting them pays on some scenes only.
-  const std::vector<uint32_t> laters = values(b->split_later, {0u, 1024u, 2048u, 4096u});
-  if ((rc = pick(laters.size(),
-                 [&](size_t i) { return gsr_batch_set_split(b, b->split_first, laters[i], b->split_backward); }, k)))
-    return rc;
-  // The first slice by the previous step's walk (its tiles mostly saturate early,
-  // so their list lengths say little); time_min's warm step fills that estimate.
-  const std::vector<uint32_t> firsts = values(b->split_first, {0u, 1024u, 2048u});
-  if ((rc = pick(firsts.size(),
-                 [&](size_t i) { return gsr_batch_set_split(b, firsts[i], b->split_later, b->split_backward); }, k)))
-    return rc;
+  // The knobs below interact (the chunk that suits the default split thresholds
+  // need not suit the chosen ones), so they are picked twice over.
+  for (int round = 0; round < 2; ++round) {
+    // Tile batch x binning chunk.
+    std::vector<std::pair<uint32_t, uint32_t>> grid = {{b->tile_batch, b->chunk}};
+    for (uint32_t ch : {1024u, 2048u, 4096u, 8192u})
+      for (uint32_t tb : {1u, 2u, 4u, 8u})
+        if (tb != b->tile_batch || ch != b->chunk) grid.push_back({tb, ch});
+    if ((rc = pick(grid.size(), [&](size_t i) { return gsr_batch_configure(b, grid[i].first, grid[i].second); }, k)))
+      return rc;
+    // The emit band: small views keep more waves resident with 128 tiles, large
+    // ones rescan their chunks less with 256.
+    std::vector<uint32_t> bands = {b->emit_band};
+    for (uint32_t eb : {128u, 256u})
+      if (eb != b->emit_band) bands.push_back(eb);
+    if ((rc = pick(bands.size(), [&](size_t i) { return gsr_batch_set_emit_band(b, bands[i]); }, k))) return rc;
+    // The walk above which the backward blends a tile as two half-tile items: low
+    // enough to shorten the launch's tail, high enough not to repeat each entry's
+    // staging for tiles that are merely long (scenes of dense clusters prefer a
+    // higher one).
+    auto values = [](uint32_t current, std::initializer_list<uint32_t> cands) {
+      std::vector<uint32_t> v = {current};
+      for (uint32_t c : cands)
+        if (c != current) v.push_back(c);
+      return v;
+    };
+    const std::vector<uint32_t> backs = values(b->split_backward, {512u, 1024u, 2048u, 4096u});
+    if ((rc = pick(backs.size(),
+                   [&](size_t i) { return gsr_batch_set_split(b, b->split_first, b->split_later, backs[i]); }, k)))
+      return rc;
+    // Whether (and from which list length) the forward's later slices split their
+    // tiles: their unsaturated tiles walk whole segments, so a few long lists can
+    // end the launch late; splitting them pays on some scenes only.
+    const std::vector<uint32_t> laters = values(b->split_later, {0u, 1024u, 2048u, 4096u});
+    if ((rc = pick(laters.size(),
+                   [&](size_t i) { return gsr_batch_set_split(b, b->split_first, laters[i], b->split_backward); }, k)))
+      return rc;
+    // The first slice by the previous step's walk (its tiles mostly saturate early,
+    // so their list lengths say little); time_min's warm step fills that estimate.
+    const std::vector<uint32_t> firsts = values(b->split_first, {0u, 1024u, 2048u});
+    if ((rc = pick(firsts.size(),
+                   [&](size_t i) { return gsr_batch_set_split(b, firsts[i], b->split_later, b->split_backward); }, k)))
+      return rc;
+  }
   if (tile_batch) *tile_batch = b->tile_batch;
   if (chunk) *chunk = b->chunk;
   return GSR_OK;
