@@ -470,8 +470,13 @@ __device__ __forceinline__ void for_each_item(const RasterArgs& a, Blend&& blend
   }
 }
 
+// Six waves per SIMD (80 VGPRs, four spilled) for the regular blend: its walk waits
+// on LDS and memory a third of its time (forward 9.34 -> 8.98 ms on the 3M-Gaussian
+// training scene, 2.62 -> 2.59 ms on the 32-view bench scene); the profiling
+// variant would spill 44 and keeps the compiler's choice.
 template <bool kLast, bool kNoCull>
-__global__ __launch_bounds__(64) void forward_persistent_kernel(RasterArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kLast ? 1 : 6))) void forward_persistent_kernel(
+    RasterArgs a) {
   __shared__ float4 lds[64][4];
   for_each_item(a, [&](uint32_t w) __attribute__((always_inline)) { forward_tile<kLast, kNoCull>(a, w, lds); });
 }
