@@ -206,6 +206,29 @@ def test_depth_slice_passes_match_full_lists(ctx, passes, growth):
     log(f"  entries full {int(e_full.sum())} sliced {int(e_sl.sum())}")
 
 
+@pytest.mark.parametrize("wide", [(4112, 64), (4112, 2176)])
+def test_depth_slices_with_wide_views(ctx, wide):
+    """Later depth slices are filtered to the Gaussians that reach an open tile
+    through a 16-bit summed-area table of each view's open tiles in LDS, for
+    batches whose views have at most 32768 tiles, and binned unfiltered beyond.
+    A 4112 x 64 view (257 x 4 tiles: the 16-bit-per-coordinate rectangles, since
+    257 columns do not fit the packed 8-bit form) is filtered; with a 4112 x 2176
+    view (257 x 136 = 34952 tiles) the batch is not.  Both render the full
+    lists' images bit for bit."""
+    sc = small_batch_scene(n=20000, views=3, seed=11, sizes=[wide, (96, 64), (160, 120)])
+    sc.gs[:, 7] += 2.0
+    log(f"views {[(c.width, c.height) for c in sc.cams]}")
+    full, e_full = run_batch(ctx, sc, graph=False, images=True, passes=1)
+    sliced, e_sl = run_batch(ctx, sc, graph=True, images=True, passes=3, growth=4, steps=2)
+    for g, s, sh, img in sliced:
+        np.testing.assert_array_equal(img, full[0][3])
+        assert rel_l2(g[:, :4], full[0][0][:, :4]) < 1e-5
+        assert rel_l2(s, full[0][1]) < 1e-5
+        assert rel_l2(sh, full[0][2]) < 1e-5
+    assert e_sl.sum() < e_full.sum()
+    log(f"  entries full {int(e_full.sum())} sliced {int(e_sl.sum())}")
+
+
 @pytest.mark.parametrize("split", [(64, 64, 64), (0, 0, 0), (0, 200, 0), (100, 0, 0)])
 def test_tile_splitting_leaves_results_unchanged(ctx, split):
     """Blending long tiles as two half-tile items (in the forward's first depth
