@@ -754,35 +754,35 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
       owner_mark[lane] = 0;
       if (cnt > 0 && start < b0 + 64 && incl > b0) owner_mark[(start > b0 ? start : b0) - b0] = lane + 1;
       __syncthreads();
+      // Every position of the batch has an owner (position 0 always has its
+      // mark), so the lanes past the last pair read valid queue slots, tiles and
+      // cursors too, and only the stores are conditional: fewer exec-mask
+      // regions (scalar instructions) per batch than branches around each step
+      // (emit 1.42 -> 1.33 ms at 256-tile bands on the 32-view bench scene,
+      // 3.96 -> 3.62 ms on the 3M-Gaussian training scene).
       const uint32_t own = wave_inclusive_scan<true>(owner_mark[lane]) - 1;
       const uint32_t e = b0 + lane;
       const bool act = e < total;
-      uint32_t t = 0, pos = 0xFFFFFFFFu, idx = 0;
-      if (act) {
-        const uint4 o = queue[own];
-        const uint32_t k = e - o.z, ow = o.y;
-        // k < the rectangle's tiles in the band (< 2^16): an approximate
-        // reciprocal and 24-bit products, then exact correction.
-        uint32_t row = uint32_t(float(k) * __builtin_amdgcn_rcpf(float(ow)));
-        while (__umul24(row, ow) > k) --row;
-        while (__umul24(row + 1, ow) <= k) ++row;
-        t = o.x + __umul24(row, tx) + (k - __umul24(row, ow));
-        idx = o.w;
-        pos = cursor[t];
-        atomicOr(reinterpret_cast<unsigned long long*>(&mask[t]), static_cast<unsigned long long>(1) << lane);
-      }
+      const uint4 o = queue[own];
+      const uint32_t k = act ? e - o.z : 0u, ow = o.y;
+      // k < the rectangle's tiles in the band (< 2^16): an approximate
+      // reciprocal, off by at most one, then one exact correction each way.
+      uint32_t row = uint32_t(float(k) * __builtin_amdgcn_rcpf(float(ow)));
+      row = __umul24(row, ow) > k ? row - 1u : row;
+      row = __umul24(row + 1u, ow) <= k ? row + 1u : row;
+      const uint32_t t = o.x + __umul24(row, tx) + (k - __umul24(row, ow));
+      const uint32_t pos = cursor[t];
+      atomicOr(reinterpret_cast<unsigned long long*>(&mask[t]),
+               act ? static_cast<unsigned long long>(1) << lane : 0ull);
       __syncthreads();
-      if (act) {
-        const uint64_t m = mask[t];
-        if (pos != 0xFFFFFFFFu) {
-          const uint64_t slot = ebase + pos + uint32_t(__popcll(m & below));
-          if (slot < capacity) order[slot] = idx;
-          else over = true;
-        }
-        if ((m >> lane) == 1) {  // highest lane of this tile in the batch
-          if (pos != 0xFFFFFFFFu) cursor[t] = pos + uint32_t(__popcll(m));
-          mask[t] = 0;
-        }
+      const uint64_t m = mask[t];
+      const bool put = act && pos != 0xFFFFFFFFu;
+      const uint64_t slot = ebase + pos + uint32_t(__popcll(m & below));
+      over |= put && slot >= capacity;
+      if (put && slot < capacity) order[slot] = o.w;
+      if (act && (m >> lane) == 1) {  // highest lane of this tile in the batch
+        cursor[t] = pos != 0xFFFFFFFFu ? pos + uint32_t(__popcll(m)) : pos;
+        mask[t] = 0;
       }
       __syncthreads();
     }
