@@ -656,7 +656,8 @@ void launch_shape(hipStream_t s, const gsr_camera* cams, uint32_t num_views, con
                   const uint64_t* valid_mask, float* stats, const uint32_t* stats_guard, int moments) {
   if (n == 0) return;
   if (!valid_mask) stats = nullptr;
-  if (moments && num_views > 1)
+  // The dense chain's queue word holds the view in its top 23 bits.
+  if (moments && num_views > 1 && num_views < (1u << 23))
     hipLaunchKernelGGL(shape_dense_kernel, dim3((n + 63) / 64), dim3(64), 0, s, cams, num_views, gs, sh, sh_degree,
                        n, grads, grads_view_stride, shape, sh_grad, grads_out, consume, touched, record, valid_mask,
                        stats, stats_guard);

@@ -206,15 +206,16 @@ def test_depth_slice_passes_match_full_lists(ctx, passes, growth):
     log(f"  entries full {int(e_full.sum())} sliced {int(e_sl.sum())}")
 
 
-@pytest.mark.parametrize("wide", [(4112, 64), (4112, 2176)])
+@pytest.mark.parametrize("wide", [(4112, 64), (4096, 2048), (4112, 2176)])
 def test_depth_slices_with_wide_views(ctx, wide):
     """Later depth slices are filtered to the Gaussians that reach an open tile
     through a 16-bit summed-area table of each view's open tiles in LDS, for
     batches whose views have at most 32768 tiles, and binned unfiltered beyond.
     A 4112 x 64 view (257 x 4 tiles: the 16-bit-per-coordinate rectangles, since
-    257 columns do not fit the packed 8-bit form) is filtered; with a 4112 x 2176
-    view (257 x 136 = 34952 tiles) the batch is not.  Both render the full
-    lists' images bit for bit."""
+    257 columns do not fit the packed 8-bit form) is filtered, and so is a 4096 x
+    2048 view (256 x 128 = 32768 tiles: the whole 64 KB of LDS for the table);
+    with a 4112 x 2176 view (257 x 136 = 34952 tiles) the batch is not.  All
+    render the full lists' images bit for bit."""
     sc = small_batch_scene(n=20000, views=3, seed=11, sizes=[wide, (96, 64), (160, 120)])
     sc.gs[:, 7] += 2.0
     log(f"views {[(c.width, c.height) for c in sc.cams]}")
