@@ -28,6 +28,18 @@ constexpr int kWaveItems = kSortChunk / (kSortThreads / 64);  // 1024 per wave
 // The empty tile rectangle in both formats (x0 = 1 > x1 = 0).
 __device__ inline uint2 empty_rect() { return make_uint2(1u, 0u); }
 
+// Workgroup L runs on die L mod 8 (eight dies, each with its own L2).  Groups of
+// 8 kDieRun workgroups give each die kDieRun consecutive logical blocks, so the
+// partial-line stores of neighbouring blocks meet in one L2 and leave it as whole
+// lines; the groups interleave the dies, which keeps their loads even, and the
+// tail maps to itself.
+constexpr uint32_t kDieRun = 8;
+__device__ inline uint32_t die_run_block(uint32_t L, uint32_t total) {
+  constexpr uint32_t group = 8 * kDieRun;
+  if (L >= total / group * group) return L;
+  return L / group * group + (L % 8) * kDieRun + (L / 8) % kDieRun;
+}
+
 __device__ inline uint64_t lanemask_lt() {
   uint32_t lane = __lane_id();
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -81,7 +93,11 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
   constexpr uint32_t kB = 1u << kBits;
   __shared__ uint32_t h[kB];
   __shared__ uint32_t rmin, rmax;
-  const uint32_t v = blockIdx.y, c = blockIdx.x;
+  // A chunk's counts sit next to its neighbouring chunks' (hist[v][digit][chunk]):
+  // consecutive chunks on one die, so they leave its L2 as whole lines (the
+  // counts were written ten times over before).
+  const uint32_t L = die_run_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const uint32_t v = L / gridDim.x, c = L % gridDim.x;
   for (uint32_t b = threadIdx.x; b < kB; b += kSortThreads) h[b] = 0;
   if (kRange && threadIdx.x == 0) {
     rmin = 0xFFFFu;
@@ -212,6 +228,12 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t L = blockIdx.x + gridDim.x * blockIdx.y, k = L / 8;
     v = L % 8 + 8 * (k / gridDim.x);
     c = k % gridDim.x;
+  } else {
+    // Each block writes its chunk as one short run per digit, next to the runs
+    // of the neighbouring chunks: consecutive chunks on one die.
+    const uint32_t L = die_run_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    v = L / gridDim.x;
+    c = L % gridDim.x;
   }
   const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
   const uint32_t lo = kOffset ? range_lo(range, v) : 0u;
@@ -659,17 +681,9 @@ constexpr uint32_t kOpenWords = 8;  // open-column mask of a band: up to 256 til
 constexpr uint32_t kLaterBandScale = 4;  // emit band of later depth slices, in first-slice bands
 constexpr uint32_t kMaxEmitBand = 4096;  // tiles; 12 B of LDS each (gsr_batch_set_emit_band's limit)
 constexpr uint32_t kEmitFixedLds = kEmitQueue * 16 + 64 * 4 + kOpenWords * 4;  // queue, owner marks, open columns
-// Workgroup L runs on die L mod 8 (eight dies, each with its own L2).  A tile's
-// slots from neighbouring chunks are neighbours in the order, so each die takes
-// kEmitDieRun consecutive chunks of a band: their short runs then meet in one L2
-// and leave it as whole lines.  Groups of 8 * kEmitDieRun workgroups interleave
-// the dies, which keeps their loads even; the tail maps to itself.
-constexpr uint32_t kEmitDieRun = 8;
-__device__ inline uint32_t emit_block(uint32_t L, uint32_t total) {
-  constexpr uint32_t group = 8 * kEmitDieRun;
-  if (L >= total / group * group) return L;
-  return L / group * group + (L % 8) * kEmitDieRun + (L / 8) % kEmitDieRun;
-}
+// A tile's slots from neighbouring chunks are neighbours in the order, so each
+// die takes kDieRun consecutive chunks of a band (die_run_block): their short
+// runs then meet in one L2 and leave it as whole lines.
 
 template <bool kPacked>
 __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict__ views, uint32_t n, uint32_t chunk,
@@ -684,7 +698,7 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
                                                       const uint32_t* __restrict__ kept) {
   extern __shared__ uint4 lds_quads[];
   const uint32_t X = gridDim.x, XY = gridDim.x * gridDim.y;
-  const uint32_t logical = emit_block(blockIdx.x + X * blockIdx.y + XY * blockIdx.z, XY * gridDim.z);
+  const uint32_t logical = die_run_block(blockIdx.x + X * blockIdx.y + XY * blockIdx.z, XY * gridDim.z);
   const uint32_t v = logical / XY, c = logical % X, lane = threadIdx.x;
   const ViewDesc vd = views[v];
   const Band bd = band_of(vd, (logical % XY) / X, emit_band);
