@@ -95,7 +95,9 @@ struct gsr_batch {
   // length says little about the walk), the forward of later slices (unsaturated
   // tiles walk their whole segment) and the backward (entries the forward walked).
   uint32_t split_first = 0, split_later = 0, split_backward = 1024;
-  uint32_t emit_band = 256;  // tiles per band of the emit kernel
+  // Tiles per band of the emit kernel: 128 (both workloads measured emit faster
+  // there than at 256 since its ranking rounds went branch-light).
+  uint32_t emit_band = 128;
   int no_cull = 0;           // verification knob: no hit-box binning or culling
   uint64_t* valid_mask = nullptr;  // [V][ceil(n / 64)]: projected validly (STATS)
   uint32_t* order = nullptr;
@@ -812,7 +814,7 @@ int gsr_batch_autotune(gsr_batch* b, const float* gs, const float* sh, const flo
   if (!b) return GSR_E_INVALID;
   // The shortest of kReps single steps (each returns after its stream is done):
   // the minimum is steadier than a mean against the host's own hiccups.
-  constexpr int kReps = 6;
+  constexpr int kReps = 8;
   // A candidate replaces the setting in effect only when it is more than 1%
   // faster, so near-ties keep the current knobs and the choice is repeatable.
   constexpr double kMargin = 0.99;

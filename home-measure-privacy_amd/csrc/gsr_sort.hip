@@ -423,7 +423,8 @@ void scan_segments(hipStream_t st, uint32_t* data, size_t seg_len, uint32_t segs
 // before) receive no entries.  The count matrix of a pass is laid out
 // [view][chunk][tile] at offset chunks * tile_base of the view.
 constexpr uint32_t kBandTiles = 2048;      // count kernel: 4 B of LDS per tile
-// The emit kernel's band (BinScratch::emit_band, 256 tiles unless tuned) costs
+// The emit kernel's band (BinScratch::emit_band: 256 tiles for the single-view
+// path, the batch's knob otherwise) costs
 // 12 B of LDS per tile: smaller bands keep more waves resident, larger ones
 // rescan each chunk's rectangles fewer times.
 

@@ -346,14 +346,14 @@ class Batch:
         self.ctx._check(self.lib.gsr_batch_set_emit_band(self.h, tiles))
 
     def set_tuning(self, tuning):
-        """Sets what tuning() returned (the emit band may be left off: 256; the
+        """Sets what tuning() returned (the emit band may be left off: 128; the
         depth slices too: the batch's own)."""
         tb, ch, sf, sl, sb = tuning[:5]
         if len(tuning) > 6:
             self.set_passes(tuning[6], tuning[7])
         self.configure(tb, ch)
         self.set_split(sf, sl, sb)
-        self.set_emit_band(tuning[5] if len(tuning) > 5 else 256)
+        self.set_emit_band(tuning[5] if len(tuning) > 5 else 128)
 
     def set_ssim_window(self, window: int):
         self.ctx._check(self.lib.gsr_batch_set_ssim_window(self.h, window))

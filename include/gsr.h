@@ -238,7 +238,7 @@ int gsr_batch_configure(gsr_batch* batch, uint32_t tile_batch, uint32_t chunk);
  * walked).  0 = never; defaults 0, 0 and 1024.  gsr_batch_autotune picks all
  * three.  Results do not depend on them. */
 int gsr_batch_set_split(gsr_batch* batch, uint32_t forward_first, uint32_t forward_later, uint32_t backward);
-/* Tiles per band of the tile-order emission (64 to 4096, default 256):
+/* Tiles per band of the tile-order emission (64 to 4096, default 128):
  * gsr_batch_autotune picks 128 or 256.  Results do not depend on it. */
 int gsr_batch_set_emit_band(gsr_batch* batch, uint32_t tiles);
 /* The knobs in effect (after gsr_batch_autotune, say), so a later run can set

@@ -158,8 +158,8 @@ def test_batch_tuning_reads_back_the_knobs(ctx):
     b.configure(0, 0)
     b.set_split(0, 0, 0)
     assert b.tuning() == (4, 2048, 0, 0, 0, 128, 1, 1)
-    b.set_tuning((1, 1024, 0, 0, 1024))  # the emit band left off: 256
-    assert b.tuning() == (1, 1024, 0, 0, 1024, 256, 1, 1)
+    b.set_tuning((1, 1024, 0, 0, 1024))  # the emit band left off: 128
+    assert b.tuning() == (1, 1024, 0, 0, 1024, 128, 1, 1)
     b.set_tuning((1, 1024, 0, 0, 1024, 256, 3, 4))
     assert b.tuning() == (1, 1024, 0, 0, 1024, 256, 3, 4)
     with pytest.raises(lib.GsrError):
