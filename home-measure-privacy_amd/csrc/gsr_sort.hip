@@ -775,7 +775,14 @@ __global__ __launch_bounds__(64) void bin_emit_kernel(const ViewDesc* __restrict
       // regions (scalar instructions) per batch than branches around each step
       // (emit 1.42 -> 1.33 ms at 256-tile bands on the 32-view bench scene,
       // 3.96 -> 3.62 ms on the 3M-Gaussian training scene).
-      const uint32_t own = wave_inclusive_scan<true>(owner_mark[lane]) - 1;
+      // Marked positions belong to consecutive entries, position 0's first: a
+      // pair's entry is position 0's plus the marks up to its position (a
+      // ballot and a lane bit count instead of a running max over the wave).
+      const uint32_t mk = owner_mark[lane];
+      const uint64_t marks = __ballot(mk != 0u);
+      const uint32_t below_marks =
+          __builtin_amdgcn_mbcnt_hi(uint32_t(marks >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(marks), 0u));
+      const uint32_t own = __builtin_amdgcn_readfirstlane(mk) - 2u + below_marks + (mk != 0u ? 1u : 0u);
       const uint32_t e = b0 + lane;
       const bool act = e < total;
       const uint4 o = queue[own];
