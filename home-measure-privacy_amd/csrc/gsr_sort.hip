@@ -325,6 +325,24 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   }
 }
 
+// Inclusive scans over the wave with row shifts and row broadcasts (DPP):
+// rows of 16 first, then row 0's total into row 1 (and row 2's into row 3),
+// then lane 31's running total into rows 2 and 3.
+template <bool kMax>
+__device__ inline uint32_t dpp_step(uint32_t x, uint32_t y) {
+  return kMax ? (x > y ? x : y) : x + y;
+}
+template <bool kMax>
+__device__ inline uint32_t wave_inclusive_scan(uint32_t x) {
+  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x111, 0xF, 0xF, true)));  // row_shr:1
+  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x112, 0xF, 0xF, true)));  // row_shr:2
+  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x114, 0xF, 0xF, true)));  // row_shr:4
+  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x118, 0xF, 0xF, true)));  // row_shr:8
+  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x142, 0xA, 0xF, false)));  // row_bcast:15
+  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x143, 0xC, 0xF, false)));  // row_bcast:31
+  return x;
+}
+
 // ---- segmented exclusive scan over equal-length segments (u32) ------------------
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
@@ -581,23 +599,31 @@ __global__ __launch_bounds__(1024) void bin_tilescan_kernel(const ViewDesc* __re
                                                             const uint32_t* __restrict__ tile_tot,
                                                             uint32_t* __restrict__ ranges,
                                                             uint64_t* __restrict__ entries) {
-  __shared__ uint32_t tmp[1024];
+  // Thread i takes tiles [i K, i K + K) in turn, after one scan of the threads'
+  // sums (wave scans, then the 16 waves' totals): one block-wide step instead of
+  // a barrier-bound scan per 1024 tiles.
+  __shared__ uint32_t wave_total[16];
   const uint32_t v = blockIdx.x;
   const ViewDesc vd = views[v];
-  const uint32_t T = vd.tiles_x * vd.tiles_y;
-  uint32_t carry = 0;
-  for (uint32_t b = 0; b < T; b += 1024) {
-    uint32_t t = b + threadIdx.x;
-    uint32_t x = (chunks > 0 && t < T) ? tile_tot[vd.tile_base + t] : 0u;
-    uint32_t total;
-    uint32_t ex = block_exclusive_scan(x, tmp, total);
-    if (t < T) {
-      ranges[size_t(vd.tile_base + t) * 2] = carry + ex;
-      ranges[size_t(vd.tile_base + t) * 2 + 1] = carry + ex + x;
-    }
-    carry += total;
+  const uint32_t T = vd.tiles_x * vd.tiles_y, K = (T + 1023) / 1024;
+  const uint32_t t0 = min(T, threadIdx.x * K), t1 = min(T, t0 + K);
+  const uint32_t* tot = tile_tot + vd.tile_base;
+  uint32_t sum = 0;
+  if (chunks > 0)
+    for (uint32_t t = t0; t < t1; ++t) sum += tot[t];
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t incl = wave_inclusive_scan<false>(sum);
+  if (__lane_id() == 63) wave_total[w] = incl;
+  __syncthreads();
+  uint32_t run = incl - sum;
+  for (uint32_t k = 0; k < w; ++k) run += wave_total[k];
+  if (threadIdx.x == 1023) entries[v] = run + sum;
+  for (uint32_t t = t0; t < t1; ++t) {
+    const uint32_t x = chunks > 0 ? tot[t] : 0u;
+    ranges[size_t(vd.tile_base + t) * 2] = run;
+    ranges[size_t(vd.tile_base + t) * 2 + 1] = run + x;
+    run += x;
   }
-  if (threadIdx.x == 0) entries[v] = carry;
 }
 
 // First order slot of each view's entries for the pass: a running total over
@@ -658,24 +684,6 @@ __global__ __launch_bounds__(256) void bin_colscan_kernel(const ViewDesc* __rest
 // mask.  Slots of a tile therefore follow (depth, index).  Done tiles carry
 // cursor 0xFFFFFFFF and are skipped; bands whose tiles are all done return at
 // once.
-
-// Inclusive scans over the wave with row shifts and row broadcasts (DPP):
-// rows of 16 first, then row 0's total into row 1 (and row 2's into row 3),
-// then lane 31's running total into rows 2 and 3.
-template <bool kMax>
-__device__ inline uint32_t dpp_step(uint32_t x, uint32_t y) {
-  return kMax ? (x > y ? x : y) : x + y;
-}
-template <bool kMax>
-__device__ inline uint32_t wave_inclusive_scan(uint32_t x) {
-  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x111, 0xF, 0xF, true)));  // row_shr:1
-  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x112, 0xF, 0xF, true)));  // row_shr:2
-  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x114, 0xF, 0xF, true)));  // row_shr:4
-  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x118, 0xF, 0xF, true)));  // row_shr:8
-  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x142, 0xA, 0xF, false)));  // row_bcast:15
-  x = dpp_step<kMax>(x, uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x143, 0xC, 0xF, false)));  // row_bcast:31
-  return x;
-}
 
 constexpr uint32_t kEmitQueue = 128;
 constexpr uint32_t kOpenWords = 8;  // open-column mask of a band: up to 256 tiles per row
